@@ -1,0 +1,162 @@
+/*
+ * yune_hip.h — C ABI of libyune_hip, the MI355X path-tracing library.
+ *
+ * It replaces the OpenCL dispatch of the reference (CLManager + RendererCore
+ * feeding kernels/legacy/udpt.cl or bdpt.cl) and its CL-GL interop with
+ * device buffers and headless HDR/PNG output.  Plain C: pointers, sizes and
+ * the structs of yune_types.h; no exceptions cross it.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative yn_status; the
+ *     message is then available from yn_last_error(ctx) (or yn_last_error(NULL)
+ *     for calls without a context).  This stands in for
+ *     CLManager::checkError's exceptions (src/CLManager.cpp:755-855) and the
+ *     bool + message of the setup* methods (src/CLManager.cpp:420-484).
+ *   - Host arrays passed in are copied; the caller keeps ownership
+ *     (CL_MEM_COPY_HOST_PTR semantics, src/CLManager.cpp:431, 455, 475).
+ *   - One context per host thread, or external serialisation.
+ *   - Images are RGBA32F, row 0 = bottom (the GL image the reference
+ *     renders into); .w holds the sample count (udpt.cl:196-210).
+ */
+#ifndef YUNE_HIP_H
+#define YUNE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "yune_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum yn_status {
+    YN_OK = 0,
+    YN_ERR_HIP = -1,          /* HIP runtime error (message names the call)        */
+    YN_ERR_INVALID = -2,      /* bad argument / malformed scene or BVH             */
+    YN_ERR_NOT_FOUND = -3,    /* unknown scene / job id, missing file              */
+    YN_ERR_NO_MEMORY = -4,    /* device allocation or memory budget exceeded       */
+    YN_ERR_IO = -5,           /* file read / write failure, bad OBJ/MTL            */
+    YN_ERR_NO_DEVICE = -6,    /* no gfx950 device                                  */
+};
+
+typedef struct yn_ctx yn_ctx;
+
+typedef struct yn_config {
+    int32_t device;           /* HIP device ordinal                                 */
+    int32_t max_paths;        /* paths in flight per batch (0: default, 8M)         */
+    size_t  mem_budget;       /* bytes for job buffers; 0: 64 GiB                   */
+    int32_t bvh_width;        /* GPU node width 2 or 4 (0: 4)                       */
+    int32_t trace_block;      /* trace kernel block size 64/128/256 (0: 128)        */
+    int32_t trace_chunk;      /* rays per wave per queue fetch (0: 64)              */
+    int32_t trace_grid;       /* trace blocks per launch (0: 8 per CU)              */
+} yn_config;
+
+/* Parameters of udpt.cl / bdpt.cl as shipped: GI on, MIS off, no bounce
+ * limit, RR after bounce 6 (BDPT: 4, 20 bounces), Lambert, background 0.4. */
+void yn_params_reference(yn_render_params* p, int32_t mode);
+
+/* ---- context: CLManager::setup / ~CLManager (src/CLManager.cpp:83-156) ---- */
+int  yn_create(const yn_config* cfg, yn_ctx** out);
+void yn_destroy(yn_ctx* ctx);
+const char* yn_last_error(yn_ctx* ctx);
+int  yn_device_info(yn_ctx* ctx, char* name, int name_len, int32_t* n_cu, size_t* hbm_bytes);
+
+/* ---- host-side scene preparation (L2 of the reference) ---- */
+/* BVH::createBVH (src/BVH.cpp:56-173) with its TriangleCPU prep and the
+ * root box of Scene::loadModel (src/Scene.cpp:351-356): bit-identical node
+ * array.  *nodes is allocated by the library; release with yn_free. */
+int  yn_bvh_build(const yn_triangle* tris, int32_t n_tri, int32_t bins,
+                  yn_bvh_node** nodes, int32_t* n_nodes);
+void yn_free(void* p);
+
+typedef struct yn_host_scene {
+    yn_triangle* tris; int32_t n_tri;
+    yn_material* mats; int32_t n_mat;
+    yn_light_quad light;
+    yn_cam cam;
+} yn_host_scene;
+/* Scene::loadModel OBJ/MTL reader (src/Scene.cpp:133-410). */
+int  yn_load_obj(const char* obj_path, yn_host_scene* out);
+/* Procedural scenes: Cornell box (36 triangles) and icosphere rooms. */
+int  yn_gen_cornell(yn_host_scene* out);
+int  yn_gen_icosphere_room(uint32_t seed, int32_t n_spheres, int32_t subdiv, yn_host_scene* out);
+void yn_host_scene_free(yn_host_scene* s);
+/* Camera::setViewMatrix + setBuffer (src/Camera.cpp:66-117, 49). */
+int  yn_camera(const float side[3], const float up[3], const float look_at[3],
+               const float eye[3], float yfov_deg, yn_cam* out);
+
+/* ---- scenes on the device: CLManager::setupVertexBuffer / setupMatBuffer /
+ *      setupBVHBuffer (src/CLManager.cpp:420-484), kernel args 3-7 ----
+ * n_node == 0 selects brute force over all triangles (arg 6 == 0,
+ * udpt.cl:278-284).  The reference's compile-time light_sources[0]
+ * (udpt.cl:97-106) is passed as `light`. */
+int  yn_scene_create(yn_ctx* ctx, const yn_triangle* tris, int32_t n_tri,
+                     const yn_material* mats, int32_t n_mat,
+                     const yn_bvh_node* nodes, int32_t n_node,
+                     const yn_light_quad* light, int32_t* scene_id);
+int  yn_scene_release(yn_ctx* ctx, int32_t scene_id);
+
+typedef struct yn_scene_info {
+    int32_t n_tri, n_ref_nodes, n_gpu_nodes, width, max_depth;
+    size_t  node_bytes, tri_bytes, shade_bytes;
+} yn_scene_info;
+int  yn_scene_info_get(yn_ctx* ctx, int32_t scene_id, yn_scene_info* out);
+
+/* ---- the kernel contract: pathtracer(args 0-13) (udpt.cl:158-160,
+ * launched by RendererCore::enqueueKernels, src/RendererCore.cpp:248-300) ----
+ * One frame over the pixels of tile `block` of a block_x x block_y grid,
+ * on caller-owned device images (out_img, in_img: W*H float4).  Scene
+ * buffers (args 3-7) are the uploaded scene; GI_CHECK, reset, rand, block,
+ * block_x, block_y keep their meaning.  Synchronous. */
+int  yn_pathtracer(yn_ctx* ctx, float* out_img, const float* in_img, const yn_cam* cam,
+                   int32_t scene_id, int32_t width, int32_t height,
+                   const yn_render_params* params, int32_t gi_check, int32_t reset,
+                   uint32_t rand, int32_t block, int32_t block_x, int32_t block_y);
+/* tonemap(current, prev, out, reset) (kernels/post-proc/tonemap.cl:14-35) on
+ * device images. */
+int  yn_tonemap(yn_ctx* ctx, const float* in_img, float* out_img, int32_t width, int32_t height);
+
+/* ---- progressive jobs: RendererCore's frame loop (src/RendererCore.cpp:
+ * 248-469, 507-579) without the GUI.  Frame f uses rand = f-th output of
+ * std::mt19937(base_seed) (the reference seeds from the clock); frame 0 of
+ * a job resets the image. ---- */
+int  yn_job_create(yn_ctx* ctx, int32_t scene_id, const yn_cam* cam, int32_t width, int32_t height,
+                   const yn_render_params* params, uint32_t base_seed, int32_t* job_id);
+int  yn_job_render(yn_ctx* ctx, int32_t job_id, int32_t n_frames);
+int  yn_job_read(yn_ctx* ctx, int32_t job_id, float* rgba);           /* W*H*4 floats  */
+int  yn_job_read_tonemapped(yn_ctx* ctx, int32_t job_id, float* rgba);
+/* RendererCore::saveImage (src/RendererCore.cpp:608-646): .hdr writes the
+ * linear image, .png the tonemapped one as 8-bit, both flipped vertically. */
+int  yn_job_save(yn_ctx* ctx, int32_t job_id, const char* path);
+int  yn_job_release(yn_ctx* ctx, int32_t job_id);
+
+typedef struct yn_stats {
+    uint64_t frames, samples;
+    uint64_t rays_ext, rays_shadow;       /* BVH queries (closest / any hit)      */
+    uint64_t nodes_ext, tris_ext;         /* counting mode only: interior nodes    */
+    uint64_t nodes_shadow, tris_shadow;   /* visited and triangles tested          */
+    uint64_t trace_launches_ext, trace_launches_shadow, iterations;
+    double   ms_total;                    /* wall time of yn_job_render calls      */
+    double   ms_trace_ext, ms_trace_shadow, ms_shade, ms_other;   /* HIP events */
+} yn_stats;
+int  yn_job_stats(yn_ctx* ctx, int32_t job_id, yn_stats* out);
+/* 1: next renders also count node visits and triangle tests (slower). */
+int  yn_job_set_counting(yn_ctx* ctx, int32_t job_id, int32_t on);
+/* Record, for the next frame rendered, each pixel's primary hit and first
+ * NEE shadow ray (for parity tests).  Arrays of W*H (shadow: W*H*8 floats
+ * o.xyz, len, d.xyz, recorded-flag). */
+int  yn_job_capture(yn_ctx* ctx, int32_t job_id, int32_t* prim_tri, int32_t* prim_light,
+                    float* prim_t, float* shadow_ray, int32_t* shadow_occ);
+
+/* ---- traceRay (udpt.cl:240-286) on caller rays: n x 8 floats
+ * (o.xyz, length, d.xyz, is_shadow).  Outputs per ray: triangle id (-1),
+ * light id (-1 / 0), final length.  Rays with is_shadow != 0 take the
+ * any-hit path; their tri is one occluder. ---- */
+int  yn_trace(yn_ctx* ctx, int32_t scene_id, const float* rays, int32_t n,
+              int32_t* tri, int32_t* light, float* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YUNE_HIP_H */

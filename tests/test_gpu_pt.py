@@ -1,0 +1,185 @@
+"""GPU parity of the HIP path tracer against the CPU oracle.
+
+The three parity bars (closest-hit ids and t, first-shadow-ray results, and
+per-pixel radiance) are measured on the same scenes, cameras and frame
+seeds on both sides.  The oracle runs here with an uncapped traversal list
+(heap_size = 0, the exact closest hit) except where a test says otherwise;
+the reference's HEAP_SIZE truncation cannot change anything on these
+scenes (checked in test_oracle_trace.py).
+
+Tolerances:
+  * closest hit: same triangle id, and the same t bit for bit, except for
+    rays whose two candidate triangles are at equal t within 1e-5 relative
+    (a tie, resolved by test order), which are counted and must be rare;
+  * shadow rays: same occluded flag, and the GPU's occluder must be a real
+    occluder according to the oracle;
+  * radiance: mean over pixels of |gpu - cpu|_1 / max(|cpu|_1, 1e-2) <= 1%.
+"""
+import numpy as np
+import pytest
+
+import yune_amd as Y
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oscene(s, nodes, heap=0):
+    return O.OScene(s.tris, s.mats, nodes, s.light, heap_size=heap)
+
+
+def ocam(cam):
+    return O.Cam.from_buffer_copy(bytes(cam))
+
+
+def random_rays(n, seed, lo, hi, shadow_frac=0.3):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 4:7] = d.astype(np.float32)
+    shadow = rng.uniform(size=n) < shadow_frac
+    rays[:, 3] = np.where(shadow, rng.uniform(0.05, 6.0, size=n), np.inf).astype(np.float32)
+    rays[:, 7] = shadow.astype(np.float32)
+    return rays
+
+
+def tri_t(s, tri_id, ray):
+    t = np.array([ray[3] if np.isfinite(ray[3]) else np.inf], np.float32)
+    tri = np.ascontiguousarray(s.tris[tri_id:tri_id + 1])
+    o = np.array([ray[0], ray[1], ray[2], 1], np.float32)
+    d = np.array([ray[4], ray[5], ray[6], 0], np.float32)
+    import ctypes as C
+    L = C.c_float(np.inf)
+    got = O.lib().or_ray_tri(O._p(tri), O._p(o), O._p(d), C.byref(L))
+    return got, L.value
+
+
+@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("scene", ["cornell", "ico"])
+def test_trace_matches_oracle(width, scene):
+    if scene == "cornell":
+        s = Y.gen_cornell()
+        lo, hi = (-0.99, -0.99, -3.99), (0.99, 0.99, -2.01)
+    else:
+        s = Y.gen_icosphere_room(11, 6, 3)
+        lo, hi = (-4.9, -4.9, -4.9), (4.9, 4.9, 4.9)
+    nodes = s.bvh()
+    rays = random_rays(20000, 5, lo, hi)
+    with Y.Context(bvh_width=width, max_paths=1 << 16) as c:
+        sid = c.scene(s, nodes)
+        gtri, glight, gt = c.trace(sid, rays)
+    otri, olight, ot, oflags = oscene(s, nodes).trace(rays)
+    shadow = rays[:, 7] != 0
+    closest = ~shadow
+    # closest hits
+    same = (gtri == otri) & (glight == olight)
+    bad = np.nonzero(closest & ~same)[0]
+    ties = 0
+    for i in bad:
+        assert gtri[i] >= 0 and otri[i] >= 0, f"ray {i}: gpu {gtri[i]}/{glight[i]} oracle {otri[i]}/{olight[i]}"
+        _, ta = tri_t(s, gtri[i], rays[i])
+        _, tb = tri_t(s, otri[i], rays[i])
+        assert abs(ta - tb) <= 1e-5 * max(ta, tb), f"ray {i}: not a tie ({ta} vs {tb})"
+        ties += 1
+    assert ties <= 0.001 * closest.sum()
+    ok = closest & same
+    assert np.array_equal(gt[ok].view(np.int32), ot[ok].view(np.int32)), "t differs"
+    # shadow rays: occlusion flag; occluder must really block
+    gocc = (gtri >= 0) | (glight >= 0)
+    oocc = (otri >= 0) | (olight >= 0)
+    assert np.array_equal(gocc[shadow], oocc[shadow])
+    for i in np.nonzero(shadow & (gtri >= 0))[0][:500]:
+        got, t = tri_t(s, gtri[i], rays[i])
+        assert got and t < rays[i, 3]
+
+
+def test_brute_force_matches_bvh():
+    s = Y.gen_cornell()
+    rays = random_rays(4000, 9, (-0.99, -0.99, -3.99), (0.99, 0.99, -2.01), shadow_frac=0.0)
+    with Y.Context(max_paths=1 << 14) as c:
+        a = c.trace(c.scene(s), rays)
+        b = c.trace(c.scene(s, brute=True), rays)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[2].view(np.int32), b[2].view(np.int32))
+
+
+def _cornell_job(c, s, W, H, p, seed):
+    sid = c.scene(s)
+    return c.job(sid, s.cam, W, H, p, base_seed=seed)
+
+
+@pytest.mark.parametrize("mis", [0, 1])
+def test_primary_and_shadow_rays_match(mis):
+    s = Y.gen_cornell()
+    W, H = 96, 80
+    p = Y.params_reference(mis=mis, max_bounces=8, rr_threshold=2)
+    with Y.Context(max_paths=1 << 16) as c:
+        job = _cornell_job(c, s, W, H, p, 3)
+        cap = job.capture_next()
+        job.render(1)
+    seeds = O.mt19937(3, 1)
+    _, ocap, _, _ = oscene(s, s.bvh(), heap=1500).render(ocam(s.cam), W, H, O.params(0, mis=mis, max_bounces=8,
+                                                         rr_threshold=2), seeds, capture=True, threads=8)
+    assert np.array_equal(cap["prim_tri"], ocap["prim_tri"])
+    assert np.array_equal(cap["prim_light"], ocap["prim_light"])
+    hit = cap["prim_tri"] >= 0
+    assert np.array_equal(cap["prim_t"][hit].view(np.int32), ocap["prim_t"][hit].view(np.int32))
+    rec = ocap["shadow_occ"] >= 0
+    assert np.array_equal(cap["shadow_occ"] >= 0, rec)
+    assert np.array_equal(cap["shadow_ray"][rec, :7].view(np.int32), ocap["shadow_ray"][rec, :7].view(np.int32))
+    assert np.array_equal(cap["shadow_occ"][rec], ocap["shadow_occ"][rec])
+    assert hit.mean() > 0.5 and rec.mean() > 0.4
+
+
+def mre(a, b):
+    a = a[..., :3].astype(np.float64)
+    b = b[..., :3].astype(np.float64)
+    fin = np.isfinite(a).all(-1) & np.isfinite(b).all(-1)
+    num = np.abs(a - b).sum(-1)
+    den = np.maximum(np.abs(b).sum(-1), 1e-2)
+    return float((num[fin] / den[fin]).mean()), float(1 - fin.mean())
+
+
+@pytest.mark.parametrize("mis,fix", [(0, 0), (1, 0), (1, 1)])
+def test_radiance_cornell(mis, fix):
+    s = Y.gen_cornell()
+    W, H, F = 64, 64, 16
+    kw = dict(mis=mis, fix_mis=fix, max_bounces=8, rr_threshold=2)
+    with Y.Context(max_paths=1 << 16) as c:
+        job = _cornell_job(c, s, W, H, Y.params_reference(**kw), 7)
+        job.render(F)
+        gimg = job.read()
+    oimg, _, _, _ = oscene(s, s.bvh()).render(ocam(s.cam), W, H, O.params(0, **kw), O.mt19937(7, F), threads=8)
+    assert np.array_equal(gimg[..., 3], oimg[..., 3])        # sample counts
+    e, nonfinite = mre(gimg, oimg)
+    assert e <= 0.01, e
+    assert nonfinite <= 0.01
+
+
+def test_radiance_icosphere_room():
+    s = Y.gen_icosphere_room(3, 9, 2)
+    W, H, F = 80, 45, 8
+    kw = dict(mis=1, max_bounces=8, rr_threshold=2)
+    with Y.Context(max_paths=1 << 16) as c:
+        sid = c.scene(s)
+        job = c.job(sid, s.cam, W, H, Y.params_reference(**kw), base_seed=11)
+        job.render(F)
+        gimg = job.read()
+    oimg, _, _, _ = oscene(s, s.bvh()).render(ocam(s.cam), W, H, O.params(0, **kw), O.mt19937(11, F), threads=8)
+    e, nonfinite = mre(gimg, oimg)
+    assert e <= 0.01, e
+
+
+def test_reference_defaults_unbounded_loop():
+    """udpt.cl as shipped: no bounce limit, RR after bounce 6, MIS off."""
+    s = Y.gen_cornell()
+    W, H, F = 32, 32, 4
+    with Y.Context(max_paths=1 << 14) as c:
+        job = _cornell_job(c, s, W, H, Y.params_reference(), 5)
+        job.render(F)
+        gimg = job.read()
+    oimg, _, _, _ = oscene(s, s.bvh()).render(ocam(s.cam), W, H, O.params(0), O.mt19937(5, F), threads=8)
+    e, _ = mre(gimg, oimg)
+    assert e <= 0.01, e

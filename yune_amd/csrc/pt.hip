@@ -1,0 +1,479 @@
+// pt.hip — generate, shade, resolve and tonemap kernels of the wavefront
+// unidirectional path tracer (kernels/legacy/udpt.cl semantics).
+//
+// The reference runs one path per work-item from camera ray to the end
+// (pathtracer -> shading -> evaluateDirectLighting, udpt.cl:158-609).  Here
+// each path is a small state machine stored in HBM; a shade pass consumes
+// the result of the path's one outstanding ray and runs until the path
+// needs its next ray (or ends):
+//
+//   PRIMARY  --hit--> [NEE start] --shadow ray--> NEE_SH --> [lobe pick,
+//   light sample, MIS direction] --MIS ray--> NEE_MIS --> [BOUNCE: sample
+//   direction] --extension ray--> BOUNCE --hit--> [emission, throughput,
+//   RR] --> NEE start at the new vertex ...
+//
+// The per-path xorshift state travels with the path and is consumed in the
+// reference's order (including the lobe pick that only happens after an
+// unoccluded shadow ray, udpt.cl:557-559), so primary and first shadow rays
+// are bit-identical to the CPU restatement and later decisions agree up to
+// libm-vs-device ulps in sin/cos/pow.
+//
+// Rays never recorded by the reference as separate entities:
+//  * the quad light is tested here when a ray is made (udpt.cl:244-276);
+//    a hit shortens the ray to the quad and marks it, so an extension ray
+//    that finds no nearer triangle is a light hit;
+//  * an NEE shadow ray the quad already blocks is occluded without a BVH
+//    query (traceRay would return true either way);
+//  * the MIS BRDF ray only matters if it reaches the light first
+//    (udpt.cl:599), so it is an any-hit query up to the quad, skipped when
+//    the quad is missed.
+#include "yn_internal_dev.h"
+#include "yn_shade.h"
+
+namespace yune {
+
+namespace {
+
+enum : int {
+    ST_DONE = 0,
+    ST_PRIMARY = 1,
+    ST_BOUNCE = 2,
+    ST_NEE_SH = 3,
+    ST_NEE_MIS = 4,
+    ST_MASK = 0xff,
+    F_DIRECT = 1 << 8,      // NEE result is the first vertex's direct light
+    F_LCAND = 1 << 9,       // pending extension ray ends on the quad
+    F_NGLOSSY = 1 << 10,    // NEE lobe: glossy
+    F_BGLOSSY = 1 << 11,    // bounce lobe: glossy
+    F_CAPSH = 1 << 12,      // first shadow ray already captured
+};
+
+constexpr float kPinkR = 0.988f, kPinkG = 0.0588f, kPinkB = 0.7529f;   // udpt.cl:110
+
+struct Out {
+    int kind = 0;           // 0 none, 1 extension, 2 shadow
+    float4 o_t;
+    float4 d_p;
+};
+
+__device__ __forceinline__ void write_sample(const DevJob& J, int slot, int pix, f4 c)
+{
+    if (any_nan(c)) c = mk(kPinkR, kPinkG, kPinkB, 1.0f);     // udpt.cl:193-194
+    J.samples[(size_t)slot * (size_t)(J.W * J.H) + (size_t)pix] = st4(c);
+}
+
+__device__ __forceinline__ Out make_ray(int kind, f4 o, f4 d, float tmax, int p)
+{
+    Out r;
+    r.kind = kind;
+    r.o_t = make_float4(o.x, o.y, o.z, tmax);
+    r.d_p = make_float4(d.x, d.y, d.z, __int_as_float(p));
+    return r;
+}
+
+// One shade step of path p.  `ev_o`, `ev_d`: the ray whose result arrived.
+__device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool from_ext, float4 ev_o, float4 ev_d)
+{
+    const PathSoA& P = a.P;
+    const int jid = P.job[p];
+    const DevJob& J = a.jobs[jid];
+    const yn_render_params& RP = J.p;
+    const DevScene& S = a.scenes[J.scene];
+    const yn_light_quad& L = S.light;
+    const bool on = RP.oren_nayar != 0;
+    const bool fix = RP.fix_mis != 0;
+
+    int st = P.state[p];
+    int code = st & ST_MASK;
+    uint32_t seed = P.seed[p];
+    const int pix = P.pix[p];
+    const int slot = P.slot[p];
+    const bool cap_on = cap.prim_tri != nullptr && slot == 0;
+
+    int tri = P.tri[p];
+    f4 hp = ld(P.hp[p]), hn = ld(P.hn[p]), rd = ld(P.rd[p]);
+    f4 tp, direct, indirect;
+    int bounce = P.bounce[p];
+
+    // NEE scratch
+    f4 wi = mk(0, 0, 0, 0);
+    float lpdf = 0.0f, nprob = 0.0f;
+    f4 ls = mk(0, 0, 0, 0);
+    f4 em = mk(0, 0, 0, 0);
+    f4 nee_res = mk(0, 0, 0, 0);
+    int occ = 0;
+
+    enum Step { S_NEE_START, S_NEE_SHADOW, S_NEE_MIS, S_NEE_DONE, S_BOUNCE_SAMPLE, S_FINISH, S_FINISH_CONST };
+    Step step;
+    f4 const_color = mk(0, 0, 0, 0);
+
+    if (code == ST_PRIMARY) {
+        float4 h = P.hit[p];
+        int htri = __float_as_int(h.y);
+        if (cap_on) {
+            cap.prim_tri[pix] = htri;
+            cap.prim_light[pix] = (htri < 0 && (st & F_LCAND)) ? 0 : -1;
+            cap.prim_t[pix] = h.x;
+        }
+        f4 d = ld(ev_d); d.w = 0.0f;
+        if (htri < 0) {
+            if (st & F_LCAND) {     // udpt.cl:440-446
+                const_color = dot(d, ld(L.normal)) < 0 ? mk(1, 1, 1, 1) : mk(0.1f, .1f, .1f, 1);
+            } else {
+                const_color = mk(RP.background[0], RP.background[1], RP.background[2], RP.background[3]);
+            }
+            step = S_FINISH_CONST;
+        } else {
+            f4 o = mk(ev_o.x, ev_o.y, ev_o.z, 1.0f);
+            tri = htri;
+            hp = o + d * h.x;
+            hn = shade_normal(S.tri_shade, tri, h.z, h.w);
+            rd = d;
+            tp = mk(1, 1, 1, 1);
+            direct = mk(0, 0, 0, 0);
+            indirect = mk(0, 0, 0, 0);
+            bounce = 0;
+            Mat m = load_mat(S.mats, S.tri_mat[tri]);
+            if (!m.spec) { st |= F_DIRECT; step = S_NEE_START; }
+            else { step = S_BOUNCE_SAMPLE; }
+        }
+    } else {
+        tp = ld(P.tp[p]);
+        direct = ld(P.direct[p]);
+        indirect = ld(P.indirect[p]);
+        if (code == ST_NEE_SH) {
+            occ = P.occ[p];
+            step = S_NEE_SHADOW;
+        } else if (code == ST_NEE_MIS) {
+            occ = P.occ[p];
+            step = S_NEE_MIS;
+        } else {   // ST_BOUNCE: extension result (udpt.cl:488-529)
+            float4 h = P.hit[p];
+            int htri = __float_as_int(h.y);
+            Mat m = load_mat(S.mats, S.tri_mat[tri]);
+            f4 nd = ld(P.nd[p]);
+            float4 bnc = P.bnc[p];
+            if (htri < 0) {
+                if ((st & F_LCAND) && m.spec) indirect = indirect + tp * ld(L.ke);
+                step = S_FINISH;
+            } else {
+                f4 o = mk(ev_o.x, ev_o.y, ev_o.z, 1.0f);
+                f4 nhp = o + nd * h.x;
+                f4 nhn = shade_normal(S.tri_shade, htri, h.z, h.w);
+                Mat nm = load_mat(S.mats, S.tri_mat[htri]);
+                indirect = indirect + tp * nm.ke;
+                if (m.spec) tp = tp * bnc.z;
+                else
+                    tp = ((tp * eval_brdf(m, nd, -rd, hp, hn, (st & F_BGLOSSY) != 0, bnc.y, on, false)) *
+                          fmaxf(dot(nd, hn), 0.0f)) / bnc.x;
+                step = S_NEE_START;
+                if (bounce > RP.rr_threshold) {    // udpt.cl:514-523
+                    float pr = fminf(luminance(tp), 0.95f);
+                    float r = next_rand(seed);
+                    if (r >= pr) step = S_FINISH;
+                    else tp = tp * (1 / pr);
+                }
+                if (step != S_FINISH) {
+                    tri = htri;
+                    hp = nhp;
+                    hn = nhn;
+                    rd = nd;
+                    bounce++;
+                    st &= ~F_DIRECT;
+                    if (nm.spec) step = S_BOUNCE_SAMPLE;
+                }
+            }
+        }
+    }
+
+    Out out;
+    bool ended = false;
+    for (int guard = 0; guard < 16 && !ended; ++guard) {
+        if (step == S_NEE_START) {   // evaluateDirectLighting (udpt.cl:535-555)
+            Mat m = load_mat(S.mats, S.tri_mat[tri]);
+            em = m.ke;
+            f4 wraw;
+            if (!sample_lights(L, hp, hn, seed, lpdf, wraw) || lpdf <= 0.0f) { nee_res = em; step = S_NEE_DONE; continue; }
+            float len = length(wraw);
+            len -= kEps * 1.5f;
+            wi = normalize(wraw);
+            bool glossy = false;
+            nprob = 0.0f;
+            if (fix) {
+                glossy = sample_glossy_pdf(m, seed, nprob, false);
+                if (nprob == 0.0f) { nee_res = em; step = S_NEE_DONE; continue; }
+                st = glossy ? (st | F_NGLOSSY) : (st & ~F_NGLOSSY);
+            }
+            f4 so = hp + wi * kEps;
+            float tq;
+            if (cap_on && !(st & F_CAPSH)) {
+                cap.shadow_o[pix] = make_float4(so.x, so.y, so.z, len);
+                cap.shadow_d[pix] = make_float4(wi.x, wi.y, wi.z, 1.0f);
+            }
+            if (quad_hit(L, so, wi, len, tq)) { occ = 1; step = S_NEE_SHADOW; continue; }
+            P.wi[p] = make_float4(wi.x, wi.y, wi.z, lpdf);
+            P.nee_prob[p] = nprob;
+            st = (st & ~ST_MASK) | ST_NEE_SH;
+            out = make_ray(2, so, wi, len, p);
+            break;
+        }
+        if (step == S_NEE_SHADOW) {  // udpt.cl:557-565 and the MIS light weight
+            if (code == ST_NEE_SH) {
+                float4 w4 = P.wi[p];
+                wi = mk(w4.x, w4.y, w4.z, 0.0f);
+                lpdf = w4.w;
+                nprob = P.nee_prob[p];
+                code = ST_DONE;     // consumed
+            }
+            if (cap_on && !(st & F_CAPSH)) { cap.shadow_occ[pix] = occ; st |= F_CAPSH; }
+            Mat m = load_mat(S.mats, S.tri_mat[tri]);
+            em = m.ke;
+            f4 w_o = -rd;
+            bool glossy = fix ? ((st & F_NGLOSSY) != 0) : false;
+            ls = mk(0, 0, 0, 0);
+            if (!occ) {
+                if (!fix) {
+                    glossy = sample_glossy_pdf(m, seed, nprob, false);
+                    if (nprob == 0.0f) { nee_res = em; step = S_NEE_DONE; continue; }
+                }
+                ls = (eval_brdf(m, wi, w_o, hp, hn, glossy, nprob, on, false) * ld(L.ke)) * fmaxf(dot(wi, hn), 0.0f);
+                ls = ls * (1 / lpdf);
+            }
+            if (!RP.mis) { nee_res = ls + em; step = S_NEE_DONE; continue; }
+            float bpdf = glossy ? phong_pdf(m, wi, w_o, hn) : cos_pdf(wi, hn);
+            float mw = power_heuristic(lpdf, lpdf, bpdf);
+            ls = ls * mw;
+            f4 bd = glossy ? phong_sample(m, bpdf, w_o, hp, hn, seed, false) : cosine_sample(bpdf, hp, hn, seed);
+            if (bpdf <= 0.0f) { nee_res = ls + em; step = S_NEE_DONE; continue; }
+            f4 bo = hp + bd * kEps;
+            float tq;
+            if (!quad_hit(L, bo, bd, INFINITY, tq)) { nee_res = ls + em; step = S_NEE_DONE; continue; }
+            st = glossy ? (st | F_NGLOSSY) : (st & ~F_NGLOSSY);
+            P.ls[p] = st4(ls);
+            P.mis[p] = make_float4(bd.x, bd.y, bd.z, bpdf);
+            P.wi[p] = make_float4(wi.x, wi.y, wi.z, lpdf);
+            P.nee_prob[p] = nprob;
+            st = (st & ~ST_MASK) | ST_NEE_MIS;
+            out = make_ray(2, bo, bd, tq, p);
+            break;
+        }
+        if (step == S_NEE_MIS) {     // udpt.cl:599-607
+            Mat m = load_mat(S.mats, S.tri_mat[tri]);
+            em = m.ke;
+            ls = ld(P.ls[p]);
+            if (occ) { nee_res = ls + em; step = S_NEE_DONE; continue; }
+            float4 mb = P.mis[p];
+            f4 bd = mk(mb.x, mb.y, mb.z, 0.0f);
+            float bpdf = mb.w;
+            lpdf = P.wi[p].w;
+            nprob = P.nee_prob[p];
+            bool glossy = (st & F_NGLOSSY) != 0;
+            float mw = power_heuristic(bpdf, lpdf, bpdf);
+            f4 bs = (eval_brdf(m, bd, -rd, hp, hn, glossy, nprob, on, false) * ld(L.ke)) * fmaxf(dot(bd, hn), 0.0f);
+            bs = bs * (mw / bpdf);
+            nee_res = (ls + bs) + em;
+            step = S_NEE_DONE;
+            continue;
+        }
+        if (step == S_NEE_DONE) {
+            if (st & F_DIRECT) direct = nee_res;
+            else indirect = indirect + tp * nee_res;
+            step = S_BOUNCE_SAMPLE;
+            continue;
+        }
+        if (step == S_BOUNCE_SAMPLE) {   // udpt.cl:458-487
+            if (!RP.gi || bounce >= RP.max_bounces) { step = S_FINISH; continue; }
+            Mat m = load_mat(S.mats, S.tri_mat[tri]);
+            float pdf = 1.0f, bprob = 0.0f, ior = 1.0f;
+            bool glossy = false;
+            f4 nd;
+            if (m.spec) nd = fresnel_sample(m, hn, -rd, ior, seed);
+            else {
+                glossy = sample_glossy_pdf(m, seed, bprob, false);
+                if (bprob == 0.0f) { step = S_FINISH; continue; }
+                nd = glossy ? phong_sample(m, pdf, -rd, hp, hn, seed, false) : cosine_sample(pdf, hp, hn, seed);
+            }
+            if (pdf <= 0.0f) { step = S_FINISH; continue; }
+            f4 no = hp + nd * kEps;
+            float tq, tmax = INFINITY;
+            st &= ~F_LCAND;
+            if (quad_hit(L, no, nd, INFINITY, tq)) { tmax = tq; st |= F_LCAND; }
+            st = glossy ? (st | F_BGLOSSY) : (st & ~F_BGLOSSY);
+            P.nd[p] = st4(nd);
+            P.bnc[p] = make_float4(pdf, bprob, ior, 0.0f);
+            st = (st & ~ST_MASK) | ST_BOUNCE;
+            out = make_ray(1, no, nd, tmax, p);
+            break;
+        }
+        if (step == S_FINISH) { write_sample(J, slot, pix, direct + indirect); st = ST_DONE; ended = true; break; }
+        if (step == S_FINISH_CONST) { write_sample(J, slot, pix, const_color); st = ST_DONE; ended = true; break; }
+    }
+    if (out.kind == 0 && !ended) {   // unreachable by construction; never leave a path hanging
+        write_sample(J, slot, pix, mk(NAN, NAN, NAN, NAN));
+        st = ST_DONE;
+    }
+
+    P.state[p] = st;
+    P.seed[p] = seed;
+    if (out.kind != 0) {
+        P.tri[p] = tri;
+        P.bounce[p] = bounce;
+        P.hp[p] = st4(hp);
+        P.hn[p] = st4(hn);
+        P.rd[p] = st4(rd);
+        P.tp[p] = st4(tp);
+        P.direct[p] = st4(direct);
+        P.indirect[p] = st4(indirect);
+    }
+    return out;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_shade_pt(ShadeArgs a, DevCapture cap)
+{
+    const uint32_t n_ext = *a.ext_in.count, n_sh = *a.sh_in.count;
+    const uint32_t total = n_ext + n_sh;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { *a.zero0 = 0; *a.zero1 = 0; }
+    const int lane = (int)__lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t base = wave * 64; base < total; base += n_waves * 64) {
+        uint32_t i = base + (uint32_t)lane;
+        Out o;
+        if (i < total) {
+            bool ext = i < n_ext;
+            uint32_t k = ext ? i : i - n_ext;
+            float4 ro = ext ? a.ext_in.o_t[k] : a.sh_in.o_t[k];
+            float4 rdp = ext ? a.ext_in.d_p[k] : a.sh_in.d_p[k];
+            o = advance(a, cap, __float_as_int(rdp.w), ext, ro, rdp);
+        }
+        uint32_t e = wave_append(a.ext_out.count, o.kind == 1);
+        if (o.kind == 1) { a.ext_out.o_t[e] = o.o_t; a.ext_out.d_p[e] = o.d_p; }
+        uint32_t s = wave_append(a.sh_out.count, o.kind == 2);
+        if (o.kind == 2) { a.sh_out.o_t[s] = o.o_t; a.sh_out.d_p[s] = o.d_p; }
+    }
+}
+
+// Primary rays (pathtracer entry, udpt.cl:162-189) for a list of segments.
+__global__ __launch_bounds__(256) void k_generate(GenArgs a, DevCapture cap)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n_paths) return;
+    int lo = 0, hi = a.n_segs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (a.segs[mid].path_base <= p) lo = mid; else hi = mid - 1;
+    }
+    const GenSeg& g = a.segs[lo];
+    int q = p - g.path_base;
+    int x = g.x0 + q % g.w, y = g.y0 + q / g.w;
+    const DevJob& J = a.jobs[g.job];
+    const DevScene& S = a.scenes[J.scene];
+    uint32_t seed = pixel_seed(g.rand, x, y, J.W);
+    float r1 = next_rand(seed);
+    float r2 = next_rand(seed);
+    f4 o, d;
+    create_ray((float)x + r1, (float)y + r2, J.W, J.H, J.cam, o, d);
+    float tq, tmax = INFINITY;
+    int st = ST_PRIMARY;
+    if (quad_hit(S.light, o, d, INFINITY, tq)) { tmax = tq; st |= F_LCAND; }
+    a.P.seed[p] = seed;
+    a.P.pix[p] = y * J.W + x;
+    a.P.job[p] = g.job;
+    a.P.slot[p] = g.slot;
+    a.P.state[p] = st;
+    a.P.rd[p] = st4(d);
+    a.ext_out.o_t[p] = make_float4(o.x, o.y, o.z, tmax);
+    a.ext_out.d_p[p] = make_float4(d.x, d.y, d.z, __int_as_float(p));
+    if (p == 0) *a.ext_out.count = (uint32_t)a.n_paths;
+    if (cap.prim_tri && g.slot == 0) {
+        cap.shadow_occ[y * J.W + x] = -1;
+        cap.shadow_d[y * J.W + x] = make_float4(0, 0, 0, 0);
+    }
+}
+
+// Running average of the batch's frames, oldest first (udpt.cl:196-210).
+__global__ __launch_bounds__(256) void k_resolve(ResolveArgs a)
+{
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.w * a.h) return;
+    const DevJob& J = a.jobs[a.job];
+    int x = a.x0 + q % a.w, y = a.y0 + q / a.w;
+    size_t pix = (size_t)y * J.W + x;
+    size_t npix = (size_t)J.W * J.H;
+    f4 acc = ld(a.accum[pix]);
+    for (int k = 0; k < a.n_slots; ++k) {
+        f4 c = ld(J.samples[(size_t)k * npix + pix]);
+        if (k == 0 && a.reset_first) {
+            c.w = 1;
+        } else {
+            int num_passes = (int)acc.w;
+            c = c + acc * (float)num_passes;
+            c = c / (float)(num_passes + 1);
+            c.w = (float)(num_passes + 1);
+        }
+        acc = c;
+    }
+    a.accum[pix] = st4(acc);
+}
+
+// tonemap.cl:14-47: Reinhard with lum_white = 1 (identity up to rounding),
+// clamp discarded as in the reference, gamma 1/2.2.
+__global__ __launch_bounds__(256) void k_tonemap(const float4* in, float4* out, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f4 col = ld(in[i]);
+    float lum_world = 0.212671f * col.x + 0.715160f * col.y + 0.072169f * col.z + 0.001f;
+    float lum_white = 1.0f;
+    float lum_display = lum_world * (1 + lum_world / (lum_white * lum_white)) / (1 + lum_world);
+    f4 sc = col / lum_world;
+    f4 ldr = mk(powf(sc.x, 1.0f), powf(sc.y, 1.0f), powf(sc.z, 1.0f), powf(sc.w, 1.0f)) * lum_display;
+    const float g = 1 / 2.2f;
+    out[i] = make_float4(powf(ldr.x, g), powf(ldr.y, g), powf(ldr.z, g), powf(ldr.w, g));
+}
+
+// Rays given by a caller (n x 8 floats: o, len, d, is_shadow): quad test
+// (traceRay, udpt.cl:244-276) and queue fill.  hit/occ slots are indexed by
+// ray number.
+__global__ __launch_bounds__(256) void k_prepare_rays(const DevScene* scene, const float* rays, int n, RayQueue q,
+                                                      int32_t* light_out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = rays + 8 * (size_t)i;
+    f4 o = mk(r[0], r[1], r[2], 1.0f), d = mk(r[4], r[5], r[6], 0.0f);
+    float len = r[3], t;
+    int light = -1;
+    if (quad_hit(scene->light, o, d, len, t)) { len = t; light = 0; }
+    light_out[i] = light;
+    q.o_t[i] = make_float4(o.x, o.y, o.z, len);
+    q.d_p[i] = make_float4(d.x, d.y, d.z, __int_as_float(i));
+    if (i == 0) *q.count = (uint32_t)n;
+}
+
+void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t s)
+{
+    int grid = (a.n_paths + 255) / 256;
+    if (grid > 0) hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, a, cap);
+}
+void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_shade_pt, dim3(grid), dim3(256), 0, s, a, cap);
+}
+void launch_resolve(const ResolveArgs& a, hipStream_t s)
+{
+    int n = a.w * a.h;
+    if (n > 0) hipLaunchKernelGGL(k_resolve, dim3((n + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_tonemap(const float4* in, float4* out, int n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n);
+}
+void launch_prepare(const DevScene* scene, const float* rays, int n, RayQueue q, int32_t* light_out, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_prepare_rays, dim3((n + 255) / 256), dim3(256), 0, s, scene, rays, n, q, light_out);
+}
+
+}  // namespace yune

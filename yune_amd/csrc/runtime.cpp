@@ -1,0 +1,866 @@
+// runtime.cpp — the C ABI (include/yune_hip.h): context, device scenes,
+// progressive jobs and the wavefront frame loop.
+//
+// Stands in for CLManager (src/CLManager.cpp: context, buffers, errors) and
+// RendererCore (src/RendererCore.cpp: per-frame seed, reset, tiling,
+// ping-pong accumulation, profiling, save).  Instead of one OpenCL launch
+// per tile per frame, a batch of frames x pixels becomes one set of paths
+// that runs through the generate / extend / shadow / shade kernels until no
+// ray is left, after which the frames are folded into the image in order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/yune_hip.h"
+#include "yn_internal_dev.h"
+
+using namespace yune;
+
+namespace {
+
+thread_local std::string g_err;     // errors of calls without a context
+}  // namespace
+
+void yune::set_global_error(const std::string& msg) { g_err = msg; }
+
+namespace {
+
+struct SceneRec {
+    bool live = false;
+    DevScene dev{};
+    void* bufs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    yn_scene_info info{};
+};
+
+struct JobRec {
+    bool live = false;
+    int32_t scene = -1;
+    yn_cam cam{};
+    int32_t W = 0, H = 0;
+    yn_render_params p{};
+    std::mt19937 engine;
+    uint64_t frame = 0;
+    float4* accum = nullptr;
+    yn_stats st{};
+    int counting = 0;
+    // pending capture (host pointers)
+    int32_t* cap_prim_tri = nullptr;
+    int32_t* cap_prim_light = nullptr;
+    float* cap_prim_t = nullptr;
+    float* cap_shadow = nullptr;
+    int32_t* cap_shadow_occ = nullptr;
+};
+
+constexpr int kMaxScenes = 64;
+constexpr int kMaxJobs = 4096;
+constexpr int kEvPool = 4096;
+
+}  // namespace
+
+struct yn_ctx {
+    int device = 0;
+    int n_cu = 0;
+    std::string name;
+    size_t hbm = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    yn_config cfg{};
+    std::vector<SceneRec> scenes;
+    std::vector<JobRec> jobs;
+    DevScene* d_scenes = nullptr;
+    DevJob* d_jobs = nullptr;
+    // wavefront buffers
+    int64_t cap_paths = 0;
+    PathSoA P{};
+    RayQueue E[2]{}, Sh[2]{};
+    DevCounters* ctr = nullptr;
+    DevCounters* h_ctr = nullptr;     // pinned mirror
+    GenSeg* d_segs = nullptr;
+    int seg_cap = 0;
+    float4* samples = nullptr;
+    std::vector<void*> allocs;
+    hipEvent_t ev[kEvPool];
+    int ev_used = 0;
+    int pt_job_slot = -1;             // job slot used by yn_pathtracer
+    size_t job_bytes = 0;
+};
+
+namespace {
+
+#define HIPCK(ctx, call)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                    \
+            return e_ == hipErrorOutOfMemory ? YN_ERR_NO_MEMORY : YN_ERR_HIP;                   \
+        }                                                                                        \
+    } while (0)
+
+int fail(yn_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg; else g_err = msg;
+    return code;
+}
+
+template <class T>
+int dalloc(yn_ctx* c, T** p, size_t count)
+{
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) {
+        c->err = std::string("hipMalloc(") + std::to_string(count * sizeof(T)) + " B): " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? YN_ERR_NO_MEMORY : YN_ERR_HIP;
+    }
+    *p = static_cast<T*>(q);
+    return 0;
+}
+
+int alloc_wavefront(yn_ctx* c, int64_t n)
+{
+    auto A = [&](auto** p) -> int {
+        int r = dalloc(c, p, (size_t)n);
+        if (r == 0) c->allocs.push_back(*p);
+        return r;
+    };
+    PathSoA& P = c->P;
+    int r = 0;
+    r |= A(&P.seed); r |= A(&P.pix); r |= A(&P.job); r |= A(&P.slot); r |= A(&P.state);
+    r |= A(&P.bounce); r |= A(&P.tri);
+    r |= A(&P.hp); r |= A(&P.hn); r |= A(&P.rd); r |= A(&P.tp); r |= A(&P.direct); r |= A(&P.indirect);
+    r |= A(&P.nd); r |= A(&P.bnc); r |= A(&P.wi); r |= A(&P.ls); r |= A(&P.mis); r |= A(&P.nee_prob);
+    r |= A(&P.hit); r |= A(&P.occ);
+    for (int k = 0; k < 2; ++k) {
+        r |= A(&c->E[k].o_t); r |= A(&c->E[k].d_p);
+        r |= A(&c->Sh[k].o_t); r |= A(&c->Sh[k].d_p);
+    }
+    r |= A(&c->samples);
+    if (r) return r < 0 ? r : YN_ERR_HIP;
+    c->cap_paths = n;
+    return 0;
+}
+
+void free_all(yn_ctx* c)
+{
+    for (void* p : c->allocs) hipFree(p);
+    c->allocs.clear();
+    for (auto& s : c->scenes)
+        for (void*& b : s.bufs) { if (b) hipFree(b); b = nullptr; }
+    for (auto& j : c->jobs) { if (j.accum) hipFree(j.accum); j.accum = nullptr; }
+}
+
+DevJob make_devjob(const JobRec& j)
+{
+    DevJob d{};
+    d.cam = j.cam;
+    d.p = j.p;
+    d.W = j.W;
+    d.H = j.H;
+    d.scene = j.scene;
+    d.samples = nullptr;
+    return d;
+}
+
+bool valid_params(const yn_render_params* p)
+{
+    return p && (p->mode == YN_MODE_PT || p->mode == YN_MODE_BDPT) && p->max_bounces >= 0 && p->bdpt_bounces >= 1;
+}
+
+bool valid_cam(const yn_cam* c)
+{
+    return c && c->r4.x == 0.0f && c->r4.y == 0.0f && c->r4.z == 0.0f && c->r4.w == 1.0f;
+}
+
+struct Region {
+    int32_t job, slot;
+    uint32_t rand;
+    int32_t x0, y0, w, h;
+};
+
+// Runs one batch of primary-ray regions to completion; samples land in
+// samples[slot][pixel] of each region's job (the sample buffer is shared:
+// job slots are laid out by the caller so regions do not overlap).
+int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, const DevCapture& cap)
+{
+    std::vector<GenSeg> segs;
+    int64_t total = 0;
+    for (const Region& r : regs) {
+        GenSeg g{r.job, r.slot, r.rand, r.x0, r.y0, r.w, r.h, (int32_t)total};
+        total += (int64_t)r.w * r.h;
+        segs.push_back(g);
+    }
+    if (total == 0) return 0;
+    if (total > c->cap_paths) return fail(c, YN_ERR_INVALID, "batch larger than the path capacity");
+    if ((int)segs.size() > c->seg_cap) {
+        if (c->d_segs) hipFree(c->d_segs);
+        c->d_segs = nullptr;
+        c->seg_cap = std::max<int>((int)segs.size(), 256);
+        int r = dalloc(c, &c->d_segs, (size_t)c->seg_cap);
+        if (r) return r;
+    }
+    hipStream_t s = c->stream;
+    HIPCK(c, hipMemcpyAsync(c->d_segs, segs.data(), sizeof(GenSeg) * segs.size(), hipMemcpyHostToDevice, s));
+    HIPCK(c, hipMemsetAsync(c->ctr, 0, sizeof(DevCounters), s));
+
+    const DevScene* scenes = c->d_scenes;
+    uint32_t* q = c->ctr->q;
+    // counters: q[0],q[1] extension queue counts, q[2],q[3] shadow queue counts,
+    // q[4],q[5] work counters of the extend / shadow launches.
+    for (int k = 0; k < 2; ++k) { c->E[k].count = q + k; c->Sh[k].count = q + 2 + k; }
+
+    GenArgs ga{};
+    ga.scenes = scenes;
+    ga.jobs = c->d_jobs;
+    ga.P = c->P;
+    ga.ext_out = c->E[0];
+    ga.segs = c->d_segs;
+    ga.n_segs = (int32_t)segs.size();
+    ga.n_paths = (int32_t)total;
+    launch_generate(ga, cap, s);
+
+    // single scene fast path
+    int single = -1;
+    {
+        int sc0 = c->jobs[regs[0].job].scene;
+        bool same = true;
+        for (const Region& r : regs) same &= c->jobs[r.job].scene == sc0;
+        if (same) single = sc0;
+    }
+    int max_depth = 0, width = 2;
+    for (const Region& r : regs) {
+        const SceneRec& sr = c->scenes[c->jobs[r.job].scene];
+        max_depth = std::max(max_depth, sr.info.max_depth);
+        width = sr.info.width;   // all scenes of a context share the width
+    }
+    int block = c->cfg.trace_block;
+    int grid = c->cfg.trace_grid > 0 ? c->cfg.trace_grid : c->n_cu * (2048 / block);
+    int shade_grid = c->n_cu * 8;
+    bool count = stat_job.counting != 0;
+
+    int ev_base = 0;
+    c->ev_used = 0;
+    auto ev = [&]() -> hipEvent_t { return c->ev[(ev_base + c->ev_used++) % kEvPool]; };
+    struct Span { hipEvent_t a, b; int kind; };
+    std::vector<Span> spans;
+
+    auto t0 = std::chrono::steady_clock::now();
+    int it = 0;
+    for (;;) {
+        const int group = 4;
+        for (int g = 0; g < group; ++g, ++it) {
+            int cur = it & 1, nxt = cur ^ 1;
+            TraceArgs ta{};
+            ta.scenes = scenes;
+            ta.jobs = c->d_jobs;
+            ta.path_job = c->P.job;
+            ta.single_scene = single;
+            ta.chunk = c->cfg.trace_chunk;
+            ta.ctr = c->ctr;
+            // extension rays
+            ta.q = c->E[cur];
+            ta.hit_out = c->P.hit;
+            ta.occ_out = nullptr;
+            ta.work = q + 4;
+            ta.zero0 = c->E[nxt].count;
+            ta.zero1 = c->Sh[nxt].count;
+            bool timed = (int)spans.size() + 3 < kEvPool / 2;
+            hipEvent_t e0 = timed ? ev() : nullptr, e1 = timed ? ev() : nullptr;
+            if (timed) hipEventRecord(e0, s);
+            if (launch_trace(ta, false, width, max_depth, block, grid, count, s) != 0)
+                return fail(c, YN_ERR_INVALID, "BVH too deep for the traversal stack");
+            if (timed) { hipEventRecord(e1, s); spans.push_back({e0, e1, 0}); }
+            // shadow rays
+            ta.q = c->Sh[cur];
+            ta.hit_out = nullptr;
+            ta.occ_out = c->P.occ;
+            ta.work = q + 5;
+            ta.zero0 = nullptr;
+            ta.zero1 = nullptr;
+            hipEvent_t e2 = timed ? ev() : nullptr, e3 = timed ? ev() : nullptr;
+            if (timed) hipEventRecord(e2, s);
+            launch_trace(ta, true, width, max_depth, block, grid, count, s);
+            if (timed) { hipEventRecord(e3, s); spans.push_back({e2, e3, 1}); }
+            // shade
+            ShadeArgs sa{};
+            sa.scenes = scenes;
+            sa.jobs = c->d_jobs;
+            sa.P = c->P;
+            sa.ext_in = c->E[cur];
+            sa.sh_in = c->Sh[cur];
+            sa.ext_out = c->E[nxt];
+            sa.sh_out = c->Sh[nxt];
+            sa.zero0 = q + 4;
+            sa.zero1 = q + 5;
+            sa.ctr = c->ctr;
+            hipEvent_t e4 = timed ? ev() : nullptr, e5 = timed ? ev() : nullptr;
+            if (timed) hipEventRecord(e4, s);
+            launch_shade_pt(sa, cap, shade_grid, s);
+            if (timed) { hipEventRecord(e5, s); spans.push_back({e4, e5, 2}); }
+        }
+        HIPCK(c, hipGetLastError());
+        HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, s));
+        HIPCK(c, hipStreamSynchronize(s));
+        int nxt = it & 1;
+        if (c->h_ctr->q[nxt] == 0 && c->h_ctr->q[2 + nxt] == 0) break;
+        if (c->h_ctr->q[7]) return fail(c, YN_ERR_INVALID, "traversal stack overflow");
+        if (it > 200000) return fail(c, YN_ERR_HIP, "wavefront loop did not drain");
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    for (const Span& sp : spans) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, sp.a, sp.b);
+        if (sp.kind == 0) stat_job.st.ms_trace_ext += ms;
+        else if (sp.kind == 1) stat_job.st.ms_trace_shadow += ms;
+        else stat_job.st.ms_shade += ms;
+    }
+    stat_job.st.iterations += (uint64_t)it;
+    stat_job.st.trace_launches_ext += (uint64_t)it;
+    stat_job.st.trace_launches_shadow += (uint64_t)it;
+    stat_job.st.rays_ext += c->h_ctr->rays_ext;
+    stat_job.st.rays_shadow += c->h_ctr->rays_shadow;
+    stat_job.st.nodes_ext += c->h_ctr->nodes_ext;
+    stat_job.st.tris_ext += c->h_ctr->tris_ext;
+    stat_job.st.nodes_shadow += c->h_ctr->nodes_shadow;
+    stat_job.st.tris_shadow += c->h_ctr->tris_shadow;
+    stat_job.st.samples += (uint64_t)total;
+    (void)t0; (void)t1;
+    return 0;
+}
+
+int upload_job_table(yn_ctx* c, int jid, float4* samples)
+{
+    DevJob d = make_devjob(c->jobs[jid]);
+    d.samples = samples;
+    HIPCK(c, hipMemcpyAsync(c->d_jobs + jid, &d, sizeof(DevJob), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+// Render n frames of rectangle [x0,x0+w) x [y0,y0+h) into `accum`, frames
+// taking seeds from `seeds`; first frame resets when reset_first.
+int render_rect(yn_ctx* c, int jid, float4* accum, int x0, int y0, int w, int h, const std::vector<uint32_t>& seeds,
+                bool reset_first, const DevCapture& cap)
+{
+    JobRec& J = c->jobs[jid];
+    int64_t npix = (int64_t)w * h;
+    if (npix == 0 || seeds.empty()) return 0;
+    int64_t full = (int64_t)J.W * J.H;
+    // Sample slots are indexed [slot][W*H]; a batch may hold K frames when
+    // K full images fit in the path capacity.
+    int64_t kcap = c->cap_paths / std::max<int64_t>(full, 1);
+    size_t f = 0;
+    while (f < seeds.size()) {
+        std::vector<Region> regs;
+        int nslots;
+        if (kcap >= 1) {
+            nslots = (int)std::min<int64_t>(kcap, (int64_t)(seeds.size() - f));
+            for (int k = 0; k < nslots; ++k) regs.push_back({jid, k, seeds[f + k], x0, y0, w, h});
+            int r = upload_job_table(c, jid, c->samples);
+            if (r) return r;
+            DevCapture cp = f == 0 ? cap : DevCapture{};
+            r = run_batch(c, regs, J, cp);
+            if (r) return r;
+            ResolveArgs ra{c->d_jobs, jid, x0, y0, w, h, nslots, (reset_first && f == 0) ? 1 : 0, accum};
+            launch_resolve(ra, c->stream);
+        } else {
+            // image larger than the path capacity: row bands, one frame at a time
+            if ((int64_t)J.W > c->cap_paths) return fail(c, YN_ERR_INVALID, "image row wider than the path capacity");
+            int rows = (int)std::max<int64_t>(1, c->cap_paths / J.W);
+            nslots = 1;
+            for (int yb = y0; yb < y0 + h; yb += rows) {
+                int hh = std::min(rows, y0 + h - yb);
+                regs.clear();
+                regs.push_back({jid, 0, seeds[f], x0, yb, w, hh});
+                // sample slot 0 is addressed as [pixel]; offset the buffer so the
+                // band's pixels land inside the path-sized sample buffer
+                float4* base = c->samples - (int64_t)yb * J.W;
+                int r = upload_job_table(c, jid, base);
+                if (r) return r;
+                DevCapture cp = f == 0 ? cap : DevCapture{};
+                r = run_batch(c, regs, J, cp);
+                if (r) return r;
+                ResolveArgs ra{c->d_jobs, jid, x0, yb, w, hh, 1, (reset_first && f == 0) ? 1 : 0, accum};
+                launch_resolve(ra, c->stream);
+            }
+        }
+        f += (size_t)nslots;
+    }
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+int check_ctx(yn_ctx* c) { return c ? 0 : fail(nullptr, YN_ERR_INVALID, "null context"); }
+int check_job(yn_ctx* c, int jid)
+{
+    if (jid < 0 || jid >= (int)c->jobs.size() || !c->jobs[jid].live) return fail(c, YN_ERR_NOT_FOUND, "unknown job");
+    return 0;
+}
+int check_scene(yn_ctx* c, int sid)
+{
+    if (sid < 0 || sid >= (int)c->scenes.size() || !c->scenes[sid].live)
+        return fail(c, YN_ERR_NOT_FOUND, "unknown scene");
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void yn_params_reference(yn_render_params* p, int32_t mode)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->mode = mode;
+    p->gi = 1;
+    p->mis = 0;                                   // udpt.cl:9
+    p->max_bounces = 100000;                      // udpt.cl:463
+    p->rr_threshold = mode == YN_MODE_BDPT ? 4 : 6;   // bdpt.cl:4, udpt.cl:6
+    p->bdpt_bounces = 20;                         // bdpt.cl:7
+    p->oren_nayar = 0;
+    p->fix_mis = 0;
+    p->background[0] = p->background[1] = p->background[2] = 0.4f;   // udpt.cl:109
+    p->background[3] = 1.0f;
+}
+
+const char* yn_last_error(yn_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int yn_create(const yn_config* cfg, yn_ctx** out)
+{
+    if (!out) return fail(nullptr, YN_ERR_INVALID, "null output");
+    *out = nullptr;
+    yn_config k{};
+    if (cfg) k = *cfg;
+    if (k.max_paths <= 0) k.max_paths = 1 << 23;
+    if (k.mem_budget == 0) k.mem_budget = (size_t)64 << 30;
+    if (k.bvh_width == 0) k.bvh_width = 4;
+    if (k.trace_block == 0) k.trace_block = 128;
+    if (k.trace_chunk <= 0) k.trace_chunk = 64;
+    if (k.bvh_width != 2 && k.bvh_width != 4) return fail(nullptr, YN_ERR_INVALID, "bvh_width must be 2 or 4");
+    if (k.trace_block != 64 && k.trace_block != 128 && k.trace_block != 256)
+        return fail(nullptr, YN_ERR_INVALID, "trace_block must be 64, 128 or 256");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= k.device || k.device < 0)
+        return fail(nullptr, YN_ERR_NO_DEVICE, "no HIP device available");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, k.device) != hipSuccess)
+        return fail(nullptr, YN_ERR_NO_DEVICE, "hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, YN_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+    yn_ctx* c = new yn_ctx();
+    c->cfg = k;
+    c->device = k.device;
+    c->n_cu = prop.multiProcessorCount;
+    c->name = prop.gcnArchName;
+    c->hbm = prop.totalGlobalMem;
+    int r = 0;
+    auto bail = [&](int code) { int rr = code; yn_destroy(c); return rr; };
+    if (hipSetDevice(k.device) != hipSuccess) { g_err = "hipSetDevice failed"; delete c; return YN_ERR_HIP; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_err = "hipStreamCreate failed"; delete c; return YN_ERR_HIP;
+    }
+    for (int i = 0; i < kEvPool; ++i) hipEventCreate(&c->ev[i]);
+    c->scenes.resize(kMaxScenes);
+    c->jobs.resize(kMaxJobs);
+    if ((r = dalloc(c, &c->d_scenes, kMaxScenes)) || (r = dalloc(c, &c->d_jobs, kMaxJobs)) || (r = dalloc(c, &c->ctr, 1))) {
+        g_err = c->err; return bail(r);
+    }
+    c->allocs.push_back(c->d_scenes);
+    c->allocs.push_back(c->d_jobs);
+    c->allocs.push_back(c->ctr);
+    if (hipHostMalloc((void**)&c->h_ctr, sizeof(DevCounters), hipHostMallocDefault) != hipSuccess) {
+        g_err = "hipHostMalloc failed"; return bail(YN_ERR_HIP);
+    }
+    if ((r = alloc_wavefront(c, k.max_paths))) { g_err = c->err; return bail(r); }
+    *out = c;
+    return 0;
+}
+
+void yn_destroy(yn_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_all(c);
+    if (c->d_segs) hipFree(c->d_segs);
+    if (c->h_ctr) hipHostFree(c->h_ctr);
+    for (int i = 0; i < kEvPool; ++i) if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int yn_device_info(yn_ctx* c, char* name, int name_len, int32_t* n_cu, size_t* hbm)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (name && name_len > 0) { std::strncpy(name, c->name.c_str(), name_len - 1); name[name_len - 1] = 0; }
+    if (n_cu) *n_cu = c->n_cu;
+    if (hbm) *hbm = c->hbm;
+    return 0;
+}
+
+int yn_scene_create(yn_ctx* c, const yn_triangle* tris, int32_t n_tri, const yn_material* mats, int32_t n_mat,
+                    const yn_bvh_node* nodes, int32_t n_node, const yn_light_quad* light, int32_t* scene_id)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (!light || !scene_id || n_mat <= 0 || !mats) return fail(c, YN_ERR_INVALID, "bad scene arguments");
+    int sid = -1;
+    for (int i = 0; i < kMaxScenes; ++i) if (!c->scenes[i].live) { sid = i; break; }
+    if (sid < 0) return fail(c, YN_ERR_NO_MEMORY, "too many scenes");
+    GpuLayout L;
+    std::string err;
+    int r = make_layout(tris, n_tri, n_mat, nodes, n_node, c->cfg.bvh_width, L, err);
+    if (r) return fail(c, r, err);
+    if (L.max_depth > 96) return fail(c, YN_ERR_INVALID, "BVH deeper than the traversal stack (96)");
+    SceneRec& S = c->scenes[sid];
+    S = SceneRec();
+    auto up = [&](int k, const void* src, size_t bytes) -> int {
+        void* d = nullptr;
+        HIPCK(c, hipMalloc(&d, std::max<size_t>(bytes, 16)));
+        S.bufs[k] = d;
+        if (bytes) HIPCK(c, hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+        return 0;
+    };
+    if ((r = up(0, L.nodes.data(), L.nodes.size() * 4)) || (r = up(1, L.tri_isect.data(), L.tri_isect.size() * 4)) ||
+        (r = up(2, L.tri_shade.data(), L.tri_shade.size() * 4)) || (r = up(3, L.tri_mat.data(), L.tri_mat.size() * 4)) ||
+        (r = up(4, mats, sizeof(yn_material) * (size_t)n_mat))) {
+        for (void*& b : S.bufs) { if (b) hipFree(b); b = nullptr; }
+        return r;
+    }
+    DevScene& D = S.dev;
+    D.nodes = (const float4*)S.bufs[0];
+    D.tri_isect = (const float4*)S.bufs[1];
+    D.tri_shade = (const float4*)S.bufs[2];
+    D.tri_mat = (const int32_t*)S.bufs[3];
+    D.mats = (const yn_material*)S.bufs[4];
+    D.light = *light;
+    // the reference constant has these w's (udpt.cl:97-104)
+    D.light.pos.w = 1.0f; D.light.normal.w = 0.0f; D.light.ke.w = 0.0f;
+    D.light.edge_l.w = 0.0f; D.light.edge_w.w = 0.0f;
+    D.root_ref = L.root_ref;
+    std::memcpy(D.root_box, L.root_box, sizeof D.root_box);
+    D.width = L.width;
+    D.brute = L.brute;
+    D.n_slots = (int32_t)L.slot_tri.size();
+    HIPCK(c, hipMemcpy(c->d_scenes + sid, &D, sizeof(DevScene), hipMemcpyHostToDevice));
+    S.info.n_tri = n_tri;
+    S.info.n_ref_nodes = n_node;
+    S.info.n_gpu_nodes = L.n_nodes;
+    S.info.width = L.width;
+    S.info.max_depth = L.max_depth;
+    S.info.node_bytes = L.nodes.size() * 4;
+    S.info.tri_bytes = L.tri_isect.size() * 4;
+    S.info.shade_bytes = L.tri_shade.size() * 4 + L.tri_mat.size() * 4;
+    S.live = true;
+    *scene_id = sid;
+    return 0;
+}
+
+int yn_scene_release(yn_ctx* c, int32_t sid)
+{
+    if (check_ctx(c) || check_scene(c, sid)) return YN_ERR_NOT_FOUND;
+    for (auto& j : c->jobs)
+        if (j.live && j.scene == sid) return fail(c, YN_ERR_INVALID, "scene still used by a job");
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    for (void*& b : c->scenes[sid].bufs) { if (b) hipFree(b); b = nullptr; }
+    c->scenes[sid].live = false;
+    return 0;
+}
+
+int yn_scene_info_get(yn_ctx* c, int32_t sid, yn_scene_info* out)
+{
+    if (check_ctx(c) || check_scene(c, sid)) return YN_ERR_NOT_FOUND;
+    if (!out) return fail(c, YN_ERR_INVALID, "null output");
+    *out = c->scenes[sid].info;
+    return 0;
+}
+
+int yn_job_create(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H, const yn_render_params* p,
+                  uint32_t base_seed, int32_t* job_id)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (check_scene(c, sid)) return YN_ERR_NOT_FOUND;
+    if (!valid_cam(cam)) return fail(c, YN_ERR_INVALID, "camera row 4 must be (0,0,0,1)");
+    if (!valid_params(p)) return fail(c, YN_ERR_INVALID, "bad render parameters");
+    if (p->mode != YN_MODE_PT) return fail(c, YN_ERR_INVALID, "BDPT mode is not available in this build");
+    if (W <= 0 || H <= 0 || (int64_t)W * H > (1ll << 28) || !job_id) return fail(c, YN_ERR_INVALID, "bad image size");
+    size_t bytes = (size_t)W * H * sizeof(float4);
+    if (c->job_bytes + bytes > c->cfg.mem_budget) return fail(c, YN_ERR_NO_MEMORY, "job memory budget exceeded");
+    int jid = -1;
+    for (int i = 0; i < kMaxJobs; ++i) if (!c->jobs[i].live && i != c->pt_job_slot) { jid = i; break; }
+    if (jid < 0) return fail(c, YN_ERR_NO_MEMORY, "too many jobs");
+    JobRec& J = c->jobs[jid];
+    J = JobRec();
+    int r = dalloc(c, &J.accum, (size_t)W * H);
+    if (r) return r;
+    HIPCK(c, hipMemset(J.accum, 0, bytes));
+    J.live = true;
+    J.scene = sid;
+    J.cam = *cam;
+    J.W = W;
+    J.H = H;
+    J.p = *p;
+    J.engine.seed(base_seed);
+    c->job_bytes += bytes;
+    *job_id = jid;
+    return upload_job_table(c, jid, c->samples);
+}
+
+int yn_job_render(yn_ctx* c, int32_t jid, int32_t n_frames)
+{
+    if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (n_frames < 0) return fail(c, YN_ERR_INVALID, "negative frame count");
+    JobRec& J = c->jobs[jid];
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<uint32_t> seeds(n_frames);
+    for (auto& s : seeds) s = (uint32_t)J.engine();
+    DevCapture cap{};
+    std::vector<void*> cap_bufs;
+    size_t npix = (size_t)J.W * J.H;
+    if (J.cap_prim_tri && n_frames > 0) {
+        int r = 0;
+        r |= dalloc(c, &cap.prim_tri, npix); cap_bufs.push_back(cap.prim_tri);
+        r |= dalloc(c, &cap.prim_light, npix); cap_bufs.push_back(cap.prim_light);
+        r |= dalloc(c, &cap.prim_t, npix); cap_bufs.push_back(cap.prim_t);
+        r |= dalloc(c, &cap.shadow_o, npix); cap_bufs.push_back(cap.shadow_o);
+        r |= dalloc(c, &cap.shadow_d, npix); cap_bufs.push_back(cap.shadow_d);
+        r |= dalloc(c, &cap.shadow_occ, npix); cap_bufs.push_back(cap.shadow_occ);
+        if (r) { for (void* b : cap_bufs) if (b) hipFree(b); return YN_ERR_NO_MEMORY; }
+    }
+    int r = render_rect(c, jid, J.accum, 0, 0, J.W, J.H, seeds, J.frame == 0, cap);
+    if (r == 0) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) r = fail(c, YN_ERR_HIP, std::string("render: ") + hipGetErrorString(e));
+    }
+    if (r == 0 && cap.prim_tri) {
+        std::vector<float4> so(npix), sd(npix);
+        hipMemcpy(J.cap_prim_tri, cap.prim_tri, npix * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(J.cap_prim_light, cap.prim_light, npix * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(J.cap_prim_t, cap.prim_t, npix * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(J.cap_shadow_occ, cap.shadow_occ, npix * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(so.data(), cap.shadow_o, npix * 16, hipMemcpyDeviceToHost);
+        hipMemcpy(sd.data(), cap.shadow_d, npix * 16, hipMemcpyDeviceToHost);
+        for (size_t i = 0; i < npix; ++i) {
+            float* q = J.cap_shadow + 8 * i;
+            q[0] = so[i].x; q[1] = so[i].y; q[2] = so[i].z; q[3] = so[i].w;
+            q[4] = sd[i].x; q[5] = sd[i].y; q[6] = sd[i].z; q[7] = sd[i].w;
+        }
+        J.cap_prim_tri = nullptr;
+    }
+    for (void* b : cap_bufs) hipFree(b);
+    if (r) return r;
+    J.frame += (uint64_t)n_frames;
+    J.st.frames += (uint64_t)n_frames;
+    J.st.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+int yn_job_capture(yn_ctx* c, int32_t jid, int32_t* prim_tri, int32_t* prim_light, float* prim_t, float* shadow_ray,
+                   int32_t* shadow_occ)
+{
+    if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (!prim_tri || !prim_light || !prim_t || !shadow_ray || !shadow_occ) return fail(c, YN_ERR_INVALID, "null capture");
+    JobRec& J = c->jobs[jid];
+    J.cap_prim_tri = prim_tri; J.cap_prim_light = prim_light; J.cap_prim_t = prim_t;
+    J.cap_shadow = shadow_ray; J.cap_shadow_occ = shadow_occ;
+    return 0;
+}
+
+int yn_job_set_counting(yn_ctx* c, int32_t jid, int32_t on)
+{
+    if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    c->jobs[jid].counting = on;
+    return 0;
+}
+
+int yn_job_stats(yn_ctx* c, int32_t jid, yn_stats* out)
+{
+    if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (!out) return fail(c, YN_ERR_INVALID, "null output");
+    *out = c->jobs[jid].st;
+    return 0;
+}
+
+int yn_job_read(yn_ctx* c, int32_t jid, float* rgba)
+{
+    if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (!rgba) return fail(c, YN_ERR_INVALID, "null output");
+    JobRec& J = c->jobs[jid];
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    HIPCK(c, hipMemcpy(rgba, J.accum, (size_t)J.W * J.H * 16, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int yn_job_read_tonemapped(yn_ctx* c, int32_t jid, float* rgba)
+{
+    if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (!rgba) return fail(c, YN_ERR_INVALID, "null output");
+    JobRec& J = c->jobs[jid];
+    size_t n = (size_t)J.W * J.H;
+    float4* tmp = nullptr;
+    int r = dalloc(c, &tmp, n);
+    if (r) return r;
+    launch_tonemap(J.accum, tmp, (int)n, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(rgba, tmp, n * 16, hipMemcpyDeviceToHost);
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(c, YN_ERR_HIP, std::string("tonemap: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int yn_job_save(yn_ctx* c, int32_t jid, const char* path)
+{
+    if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (!path) return fail(c, YN_ERR_INVALID, "null path");
+    JobRec& J = c->jobs[jid];
+    std::string p(path);
+    size_t n = (size_t)J.W * J.H;
+    std::vector<float> img(n * 4);
+    bool hdr = p.size() >= 4 && p.compare(p.size() - 4, 4, ".hdr") == 0;
+    int r = hdr ? yn_job_read(c, jid, img.data()) : yn_job_read_tonemapped(c, jid, img.data());
+    if (r) return r;
+    // stbi_flip_vertically_on_write(1) (RendererCore.cpp:615): file row 0 is
+    // image row H-1.
+    if (hdr) {
+        std::vector<float> rgb(n * 3);
+        for (int y = 0; y < J.H; ++y)
+            for (int x = 0; x < J.W; ++x)
+                for (int k = 0; k < 3; ++k)
+                    rgb[((size_t)y * J.W + x) * 3 + k] = img[((size_t)(J.H - 1 - y) * J.W + x) * 4 + k];
+        if (write_hdr(p, J.W, J.H, rgb.data())) return fail(c, YN_ERR_IO, "cannot write " + p);
+    } else {
+        std::vector<uint8_t> rgb(n * 3);
+        for (int y = 0; y < J.H; ++y)
+            for (int x = 0; x < J.W; ++x)
+                for (int k = 0; k < 3; ++k) {
+                    float v = img[((size_t)(J.H - 1 - y) * J.W + x) * 4 + k];
+                    v = std::isnan(v) ? 0.0f : std::min(1.0f, std::max(0.0f, v));
+                    rgb[((size_t)y * J.W + x) * 3 + k] = (uint8_t)std::lround(v * 255.0f);
+                }
+        if (write_png(p, J.W, J.H, rgb.data())) return fail(c, YN_ERR_IO, "cannot write " + p);
+    }
+    return 0;
+}
+
+int yn_job_release(yn_ctx* c, int32_t jid)
+{
+    if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    JobRec& J = c->jobs[jid];
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if (J.accum) hipFree(J.accum);
+    c->job_bytes -= (size_t)J.W * J.H * sizeof(float4);
+    J = JobRec();
+    return 0;
+}
+
+int yn_pathtracer(yn_ctx* c, float* out_img, const float* in_img, const yn_cam* cam, int32_t sid, int32_t W, int32_t H,
+                  const yn_render_params* params, int32_t gi_check, int32_t reset, uint32_t rand, int32_t block,
+                  int32_t block_x, int32_t block_y)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (check_scene(c, sid)) return YN_ERR_NOT_FOUND;
+    if (!out_img || (!in_img && !reset)) return fail(c, YN_ERR_INVALID, "null image");
+    if (!valid_cam(cam)) return fail(c, YN_ERR_INVALID, "camera row 4 must be (0,0,0,1)");
+    if (!valid_params(params) || params->mode != YN_MODE_PT) return fail(c, YN_ERR_INVALID, "bad render parameters");
+    if (W <= 0 || H <= 0 || block_x <= 0 || block_y <= 0 || block < 0 || block >= block_x * block_y)
+        return fail(c, YN_ERR_INVALID, "bad image or block arguments");
+    // tile of `block` (udpt.cl:166-170)
+    int tw = (int)std::ceil((float)W / block_x), th = (int)std::ceil((float)H / block_y);
+    int x0 = tw * (block % block_x), y0 = th * (block / block_x);
+    int w = std::min(tw, W - x0), h = std::min(th, H - y0);
+    if (w <= 0 || h <= 0) return 0;
+    if (c->pt_job_slot < 0) c->pt_job_slot = kMaxJobs - 1;
+    int jid = c->pt_job_slot;
+    JobRec& J = c->jobs[jid];
+    J = JobRec();
+    J.scene = sid; J.cam = *cam; J.W = W; J.H = H; J.p = *params; J.p.gi = gi_check;
+    // previous frame -> output for the tile, then fold this frame in place
+    if (!reset)
+        HIPCK(c, hipMemcpy2DAsync(out_img + 4 * ((size_t)y0 * W + x0), (size_t)W * 16, in_img + 4 * ((size_t)y0 * W + x0),
+                                  (size_t)W * 16, (size_t)w * 16, (size_t)h, hipMemcpyDeviceToDevice, c->stream));
+    std::vector<uint32_t> seeds(1, rand);
+    int r = render_rect(c, jid, (float4*)out_img, x0, y0, w, h, seeds, reset != 0, DevCapture{});
+    if (r == 0) HIPCK(c, hipStreamSynchronize(c->stream));
+    return r;
+}
+
+int yn_tonemap(yn_ctx* c, const float* in_img, float* out_img, int32_t W, int32_t H)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (!in_img || !out_img || W <= 0 || H <= 0) return fail(c, YN_ERR_INVALID, "bad tonemap arguments");
+    launch_tonemap((const float4*)in_img, (float4*)out_img, W * H, c->stream);
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri, int32_t* light, float* t)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (check_scene(c, sid)) return YN_ERR_NOT_FOUND;
+    if (n < 0 || (n > 0 && (!rays || !tri || !light || !t))) return fail(c, YN_ERR_INVALID, "bad trace arguments");
+    if (n == 0) return 0;
+    if (n > c->cap_paths) return fail(c, YN_ERR_INVALID, "more rays than the path capacity");
+    hipStream_t s = c->stream;
+    std::vector<int> idx[2];
+    for (int i = 0; i < n; ++i) idx[rays[8 * (size_t)i + 7] != 0.0f ? 1 : 0].push_back(i);
+    float* d_rays = nullptr;
+    int32_t* d_light = nullptr;
+    int r = dalloc(c, &d_rays, (size_t)n * 8);
+    if (r) return r;
+    r = dalloc(c, &d_light, (size_t)n);
+    if (r) { hipFree(d_rays); return r; }
+    std::vector<float> sub;
+    std::vector<float4> hits(n);
+    std::vector<int32_t> lights(n), occ(n);
+    const SceneRec& S = c->scenes[sid];
+    for (int any = 0; any < 2; ++any) {
+        int m = (int)idx[any].size();
+        if (m == 0) continue;
+        sub.resize((size_t)m * 8);
+        for (int k = 0; k < m; ++k) std::memcpy(&sub[8 * (size_t)k], rays + 8 * (size_t)idx[any][k], 32);
+        HIPCK(c, hipMemcpyAsync(d_rays, sub.data(), sub.size() * 4, hipMemcpyHostToDevice, s));
+        HIPCK(c, hipMemsetAsync(c->ctr, 0, sizeof(DevCounters), s));
+        RayQueue q = c->E[0];
+        q.count = c->ctr->q;
+        launch_prepare(c->d_scenes + sid, d_rays, m, q, d_light, s);
+        TraceArgs ta{};
+        ta.scenes = c->d_scenes;
+        ta.jobs = c->d_jobs;
+        ta.path_job = c->P.job;
+        ta.single_scene = sid;
+        ta.chunk = c->cfg.trace_chunk;
+        ta.ctr = c->ctr;
+        ta.q = q;
+        ta.hit_out = c->P.hit;
+        ta.occ_out = any ? c->P.occ : nullptr;
+        ta.work = c->ctr->q + 4;
+        int grid = c->n_cu * (2048 / c->cfg.trace_block);
+        if (launch_trace(ta, any != 0, S.info.width, S.info.max_depth, c->cfg.trace_block, grid, false, s) != 0)
+            return fail(c, YN_ERR_INVALID, "BVH too deep");
+        HIPCK(c, hipGetLastError());
+        HIPCK(c, hipMemcpyAsync(hits.data(), c->P.hit, (size_t)m * 16, hipMemcpyDeviceToHost, s));
+        HIPCK(c, hipMemcpyAsync(lights.data(), d_light, (size_t)m * 4, hipMemcpyDeviceToHost, s));
+        HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, s));
+        HIPCK(c, hipStreamSynchronize(s));
+        if (c->h_ctr->q[7]) return fail(c, YN_ERR_INVALID, "traversal stack overflow");
+        for (int k = 0; k < m; ++k) {
+            int i = idx[any][k];
+            int tr;
+            std::memcpy(&tr, &hits[k].y, 4);
+            tri[i] = tr;
+            t[i] = hits[k].x;
+            // a triangle nearer than the quad replaces the light hit (udpt.cl:386)
+            light[i] = tr >= 0 ? -1 : lights[k];
+        }
+    }
+    hipFree(d_rays);
+    hipFree(d_light);
+    return 0;
+}
+
+}  // extern "C"

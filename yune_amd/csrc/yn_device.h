@@ -1,0 +1,86 @@
+// yn_device.h — device-side data shared by the HIP kernels and the host
+// runtime that launches them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/yune_types.h"
+
+namespace yune {
+
+constexpr float kPI = 3.14159265359f;       // udpt.cl:3
+constexpr float kInvPI = 0.31830988618f;    // udpt.cl:4
+constexpr float kEps = 0.0001f;             // udpt.cl:5
+
+// One uploaded scene in its GPU layout.
+struct DevScene {
+    const float4* nodes;        // width 2: 4 float4 per node, width 4: 8 float4 per node
+    const float4* tri_isect;    // 3 float4 per slot: v1 (w = tri id bits), e1, e2
+    const float4* tri_shade;    // 3 float4 per triangle: normalised vertex normals
+    const int32_t* tri_mat;     // matID per triangle
+    const yn_material* mats;
+    yn_light_quad light;
+    uint32_t root_ref;
+    float root_box[6];
+    int32_t width;              // 2 or 4
+    int32_t brute;              // 1: no BVH, test every slot in order
+    int32_t n_slots;
+    int32_t pad;
+};
+
+// Per-job constants (a job = one image being accumulated).
+struct DevJob {
+    yn_cam cam;
+    yn_render_params p;
+    int32_t W, H;
+    int32_t scene;              // index into the scene table
+    int32_t pad;
+    float4* samples;            // [slots][W*H] one sample colour per frame slot
+};
+
+// A queue of rays: rays[i] = {o.xyz, tmax}, {d.xyz, path index bits}.
+struct RayQueue {
+    float4* o_t;
+    float4* d_p;
+    uint32_t* count;            // number of rays in the queue
+};
+
+// Path states, structure of arrays, one entry per path in flight.
+struct PathSoA {
+    uint32_t* seed;
+    int32_t* pix;               // pixel index y*W + x
+    int32_t* job;               // job slot
+    int32_t* slot;              // frame slot within the job's sample buffer
+    int32_t* state;             // state | flags
+    int32_t* bounce;            // bounce index i (udpt.cl:463)
+    int32_t* tri;               // triangle of the current vertex
+    float4* hp;                 // current vertex position
+    float4* hn;                 // current vertex shading normal
+    float4* rd;                 // direction of the ray that found the current vertex
+    float4* tp;                 // throughput
+    float4* direct;
+    float4* indirect;
+    float4* nd;                 // pending bounce direction
+    float4* bnc;                // pending bounce: pdf, brdf_prob, ior_factor, -
+    float4* wi;                 // NEE: normalised w_i, w = light pdf
+    float4* ls;                 // NEE: light sample (MIS-weighted)
+    float4* mis;                // NEE: MIS brdf dir xyz, w = brdf pdf
+    float* nee_prob;            // NEE: lobe probability of the light sample
+    float4* hit;                // closest-hit result: t, tri bits, u, v
+    int32_t* occ;               // any-hit result
+};
+
+// Counters and stats in device memory.
+struct DevCounters {
+    uint32_t q[8];              // queue counts, work counters (see runtime)
+    unsigned long long rays_ext;
+    unsigned long long rays_shadow;
+    unsigned long long nodes_ext;     // counting mode only: interior nodes visited
+    unsigned long long tris_ext;      // counting mode only: triangle tests
+    unsigned long long nodes_shadow;
+    unsigned long long tris_shadow;
+    unsigned long long pad[2];
+};
+
+}  // namespace yune

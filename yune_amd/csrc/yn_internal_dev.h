@@ -1,0 +1,106 @@
+// yn_internal_dev.h — kernel argument blocks and wave helpers shared by the
+// .hip translation units and the host runtime.
+#pragma once
+
+#include "yn_device.h"
+#include "yn_internal.h"
+#include "yn_math.h"
+
+namespace yune {
+
+struct TraceArgs {
+    const DevScene* scenes;     // scene table (device)
+    const DevJob* jobs;         // job table (device)
+    const int32_t* path_job;    // per path: job slot
+    int32_t single_scene;       // >= 0: every ray uses this scene
+    int32_t chunk;              // rays taken per wave per atomic
+    RayQueue q;
+    float4* hit_out;            // per path: t, tri bits, u, v
+    int32_t* occ_out;           // per path (any-hit)
+    uint32_t* work;             // work counter, zero at launch
+    uint32_t* zero0;            // counters this launch clears (next queues)
+    uint32_t* zero1;
+    DevCounters* ctr;
+};
+
+// Segment of primary rays: a rectangle of one job's image at one frame.
+struct GenSeg {
+    int32_t job, slot;
+    uint32_t rand;              // frame seed (kernel arg 10)
+    int32_t x0, y0, w, h;
+    int32_t path_base;          // first path index of the segment
+};
+
+struct ShadeArgs {
+    const DevScene* scenes;
+    const DevJob* jobs;
+    PathSoA P;
+    RayQueue ext_in, sh_in;     // results to consume
+    RayQueue ext_out, sh_out;   // rays to produce
+    uint32_t* zero0;            // work counters to clear for the next traces
+    uint32_t* zero1;
+    DevCounters* ctr;
+};
+
+struct GenArgs {
+    const DevScene* scenes;
+    const DevJob* jobs;
+    PathSoA P;
+    RayQueue ext_out;
+    const GenSeg* segs;
+    int32_t n_segs;
+    int32_t n_paths;
+};
+
+struct ResolveArgs {
+    DevJob* jobs;
+    int32_t job;
+    int32_t x0, y0, w, h;
+    int32_t n_slots;            // frames in this batch, oldest first
+    int32_t reset_first;        // first frame of the batch overwrites (kernel arg 9)
+    float4* accum;              // the job's RGBA32F image, .w = count
+};
+
+// Capture of the first frame for hit-ID parity (null pointers: off).
+struct DevCapture {
+    int32_t* prim_tri;
+    int32_t* prim_light;
+    float* prim_t;
+    float4* shadow_o;           // xyz origin, w = len
+    float4* shadow_d;           // xyz dir, w = 1 when recorded
+    int32_t* shadow_occ;
+};
+
+#if defined(__HIPCC__)
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Wave-aggregated append: one atomic per wave, lanes get consecutive slots
+// in lane order.  Every active lane of the wave must call it.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
+{
+    unsigned long long mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const int lane = (int)__lane_id();
+    uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + prefix;
+}
+#endif
+
+int launch_trace(const TraceArgs& a, bool any_hit, int width, int stack, int block, int grid, bool count,
+                 hipStream_t stream);
+void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t stream);
+void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, hipStream_t stream);
+void launch_resolve(const ResolveArgs& a, hipStream_t stream);
+void launch_tonemap(const float4* in, float4* out, int n, hipStream_t stream);
+void launch_prepare(const DevScene* scene, const float* rays, int n, RayQueue q, int32_t* light_out,
+                    hipStream_t stream);
+
+}  // namespace yune
