@@ -50,6 +50,10 @@ typedef struct yn_config {
     int32_t trace_block;      /* trace kernel block size 64/128/256 (0: 128)        */
     int32_t trace_chunk;      /* rays per wave per queue fetch (0: 64)              */
     int32_t trace_grid;       /* trace blocks per launch (0: 8 per CU)              */
+    float   hoist_frac;       /* triangles whose box area exceeds this fraction of the
+                                 root box's are tested by every ray instead of living
+                                 in the tree (0: 0.01; < 0: off)                    */
+    int32_t leaf_max;         /* leaves of more triangles are split (1..10; 0: 4)   */
 } yn_config;
 
 /* Parameters of udpt.cl / bdpt.cl as shipped: GI on, MIS off, no bounce
@@ -98,7 +102,7 @@ int  yn_scene_create(yn_ctx* ctx, const yn_triangle* tris, int32_t n_tri,
 int  yn_scene_release(yn_ctx* ctx, int32_t scene_id);
 
 typedef struct yn_scene_info {
-    int32_t n_tri, n_ref_nodes, n_gpu_nodes, width, max_depth;
+    int32_t n_tri, n_ref_nodes, n_gpu_nodes, width, max_depth, n_hoist;
     size_t  node_bytes, tri_bytes, shade_bytes;
 } yn_scene_info;
 int  yn_scene_info_get(yn_ctx* ctx, int32_t scene_id, yn_scene_info* out);
@@ -143,9 +147,10 @@ typedef struct yn_stats {
 int  yn_job_stats(yn_ctx* ctx, int32_t job_id, yn_stats* out);
 /* 1: next renders also count node visits and triangle tests (slower). */
 int  yn_job_set_counting(yn_ctx* ctx, int32_t job_id, int32_t on);
-/* Record, for the next frame rendered, each pixel's primary hit and first
- * NEE shadow ray (for parity tests).  Arrays of W*H (shadow: W*H*8 floats
- * o.xyz, len, d.xyz, recorded-flag). */
+/* Record, for the next frame rendered, each pixel's primary hit and the
+ * NEE shadow ray cast from it (for parity tests).  Arrays of W*H (shadow:
+ * W*H*8 floats o.xyz, len, d.xyz, recorded-flag; occ -1 when the primary
+ * vertex cast no shadow ray). */
 int  yn_job_capture(yn_ctx* ctx, int32_t job_id, int32_t* prim_tri, int32_t* prim_light,
                     float* prim_t, float* shadow_ray, int32_t* shadow_occ);
 

@@ -895,6 +895,7 @@ static v4 shade_pt(tctx* c, const yn_render_params* P, ray_t ray, uint32_t* seed
     }
     v4 direct = V(0, 0, 0, 0), indirect = V(0, 0, 0, 0), throughput = V(1, 1, 1, 1);
     int matID = s->tris[hit.tri].matID;
+    /* the capture records the NEE shadow ray of the primary vertex only */
     if (!s->mats[matID].is_specular) direct = edl(c, P, neg(ray.d), &hit, seed, 0, ec);
     if (P->gi) {
         for (int i = 0; i < P->max_bounces; i++) {
@@ -930,7 +931,7 @@ static v4 shade_pt(tctx* c, const yn_render_params* P, ray_t ray, uint32_t* seed
             }
             matID = s->tris[nh.tri].matID;
             if (!s->mats[matID].is_specular)
-                indirect = add(indirect, mul(throughput, edl(c, P, neg(nr.d), &nh, seed, 0, ec)));
+                indirect = add(indirect, mul(throughput, edl(c, P, neg(nr.d), &nh, seed, 0, NULL)));
             hit = nh;
             ray = nr;
         }
@@ -1070,7 +1071,8 @@ static v4 shade_bdpt(tctx* c, const yn_render_params* P, ray_t ray, uint32_t* se
         v4 spec = F4(m->ks), emission = F4(m->ke);
         ks = fmaxf(fmaxf(spec.x, fmaxf(spec.y, spec.z)), 0.1f);
         throughput = ep[i].contrib;
-        color = add(color, muls(mul(throughput, add(emission, edl(c, P, neg(ep[i].dir), &ep[i].hit, seed, 1, ec))),
+        color = add(color, muls(mul(throughput, add(emission, edl(c, P, neg(ep[i].dir), &ep[i].hit, seed, 1,
+                                                                 i == 0 ? ec : NULL))),
                                 eye_path_weight));
         for (int j = lp_len - 1; j > 0; j--) {
             ray_t dr;

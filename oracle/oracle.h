@@ -79,7 +79,8 @@ typedef struct or_capture {
     int32_t* prim_tri;      /* W*H: primary hit triangle (-1 none)          */
     int32_t* prim_light;    /* W*H: primary hit light (-1 none)             */
     float*   prim_t;        /* W*H: primary hit distance                    */
-    float*   shadow_ray;    /* W*H*8: first NEE shadow ray o,len,d,valid    */
+    float*   shadow_ray;    /* W*H*8: the primary vertex's NEE shadow ray:
+                               o.xyz, len, d.xyz, 1                          */
     int32_t* shadow_occ;    /* W*H: its result (1 occluded, 0 visible, -1 none) */
     int32_t* trunc;         /* W*H: bit0 primary truncated, bit1 shadow truncated */
 } or_capture;

@@ -57,9 +57,10 @@ def tri_t(s, tri_id, ray):
     return got, L.value
 
 
+@pytest.mark.parametrize("hoist,leaf_max", [(0.01, 4), (-1.0, 10), (0.01, 1)])
 @pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("scene", ["cornell", "ico"])
-def test_trace_matches_oracle(width, scene):
+def test_trace_matches_oracle(width, scene, hoist, leaf_max):
     if scene == "cornell":
         s = Y.gen_cornell()
         lo, hi = (-0.99, -0.99, -3.99), (0.99, 0.99, -2.01)
@@ -68,8 +69,9 @@ def test_trace_matches_oracle(width, scene):
         lo, hi = (-4.9, -4.9, -4.9), (4.9, 4.9, 4.9)
     nodes = s.bvh()
     rays = random_rays(20000, 5, lo, hi)
-    with Y.Context(bvh_width=width, max_paths=1 << 16) as c:
+    with Y.Context(bvh_width=width, max_paths=1 << 16, hoist_frac=hoist, leaf_max=leaf_max) as c:
         sid = c.scene(s, nodes)
+        assert (c.scene_info(sid).n_hoist > 0) == (hoist > 0)
         gtri, glight, gt = c.trace(sid, rays)
     otri, olight, ot, oflags = oscene(s, nodes).trace(rays)
     shadow = rays[:, 7] != 0
@@ -84,7 +86,9 @@ def test_trace_matches_oracle(width, scene):
         _, tb = tri_t(s, otri[i], rays[i])
         assert abs(ta - tb) <= 1e-5 * max(ta, tb), f"ray {i}: not a tie ({ta} vs {tb})"
         ties += 1
-    assert ties <= 0.001 * closest.sum()
+    # every mismatch is a verified tie; ties come from triangles sharing an
+    # edge or a plane and must stay a small minority
+    assert ties <= 0.005 * closest.sum()
     ok = closest & same
     assert np.array_equal(gt[ok].view(np.int32), ot[ok].view(np.int32)), "t differs"
     # shadow rays: occlusion flag; occluder must really block
@@ -128,7 +132,8 @@ def test_primary_and_shadow_rays_match(mis):
     assert np.array_equal(cap["prim_t"][hit].view(np.int32), ocap["prim_t"][hit].view(np.int32))
     rec = ocap["shadow_occ"] >= 0
     assert np.array_equal(cap["shadow_occ"] >= 0, rec)
-    assert np.array_equal(cap["shadow_ray"][rec, :7].view(np.int32), ocap["shadow_ray"][rec, :7].view(np.int32))
+    diff = cap["shadow_ray"][rec, :7].view(np.int32) != ocap["shadow_ray"][rec, :7].view(np.int32)
+    assert not diff.any(), f"shadow rays differ: {diff.sum(0)} of {rec.sum()}"
     assert np.array_equal(cap["shadow_occ"][rec], ocap["shadow_occ"][rec])
     assert hit.mean() > 0.5 and rec.mean() > 0.4
 
@@ -183,3 +188,69 @@ def test_reference_defaults_unbounded_loop():
     oimg, _, _, _ = oscene(s, s.bvh()).render(ocam(s.cam), W, H, O.params(0), O.mt19937(5, F), threads=8)
     e, _ = mre(gimg, oimg)
     assert e <= 0.01, e
+
+
+def comb_scene(n):
+    """n small triangles along the -z axis, under a hand-made comb-shaped
+    BVH (every interior node: one leaf child and one interior child), so
+    a ray down the axis keeps one deferred child per level on its stack:
+    deeper than the LDS part of the traversal stack."""
+    s = Y.gen_cornell()
+    tris = np.zeros(n, dtype=Y.TRI_DTYPE)
+    for i in range(n):
+        z = -1.0 - 0.1 * i
+        tris[i]["v1"] = (-0.05, -0.05, z, 1)
+        tris[i]["v2"] = (0.05, -0.05, z, 1)
+        tris[i]["v3"] = (0.0, 0.05, z, 1)
+        for k in ("vn1", "vn2", "vn3"):
+            tris[i][k] = (0, 0, 1, 0)
+        tris[i]["matID"] = 0
+    def tri_box(i):
+        v = np.stack([tris[i]["v1"][:3], tris[i]["v2"][:3], tris[i]["v3"][:3]])
+        lo, hi = v.min(0), v.max(0)
+        hi = np.where(hi - lo == 0, hi + np.float32(0.2), hi)
+        return lo, hi
+    # interior node j (index 2j, j < n-1): children 2j+1, a leaf holding the
+    # far triangle n-1-j, and 2j+2, the next interior node (the last index
+    # is the leaf of the nearest triangle 0)
+    out = np.zeros(2 * n - 1, dtype=Y.NODE_DTYPE)
+    out["vert_list"] = -1
+    for j in range(n - 1):
+        out[2 * j]["child_idx"] = 2 * j + 1
+        out[2 * j]["vert_len"] = -1
+        out[2 * j + 1]["child_idx"] = -1
+        out[2 * j + 1]["vert_len"] = 1
+        out[2 * j + 1]["vert_list"][0] = n - 1 - j
+    out[2 * (n - 1)]["child_idx"] = -1
+    out[2 * (n - 1)]["vert_len"] = 1
+    out[2 * (n - 1)]["vert_list"][0] = 0
+    for j in range(2 * n - 2, -1, -1):
+        nd = out[j]
+        if nd["vert_len"] > 0:
+            lo, hi = tri_box(nd["vert_list"][0])
+        else:
+            c = nd["child_idx"]
+            lo = np.minimum(out[c]["p_min"][:3], out[c + 1]["p_min"][:3])
+            hi = np.maximum(out[c]["p_max"][:3], out[c + 1]["p_max"][:3])
+        out[j]["p_min"] = (*lo, 1)
+        out[j]["p_max"] = (*hi, 1)
+    s.tris, s.nodes = tris, out
+    s.mats = s.mats[:1]
+    return s
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_deep_tree_uses_stack_overflow_area(width):
+    s = comb_scene(60)
+    rays = np.zeros((64, 8), np.float32)
+    rays[:, 2] = 0.0
+    rays[:, 0] = np.linspace(-0.01, 0.01, 64)
+    rays[:, 3] = np.inf
+    rays[:, 6] = -1.0
+    with Y.Context(bvh_width=width, max_paths=1 << 12, hoist_frac=-1, leaf_max=10) as c:
+        sid = c.scene(s, s.nodes)
+        assert c.scene_info(sid).max_depth > 16
+        gtri, glight, gt = c.trace(sid, rays)
+    otri, olight, ot, _ = oscene(s, s.nodes).trace(rays)
+    assert np.array_equal(gtri, otri) and np.array_equal(gt.view(np.int32), ot.view(np.int32))
+    assert (gtri == 0).all()

@@ -48,7 +48,7 @@ class RenderParams(C.Structure):
 class Config(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_paths", C.c_int32), ("mem_budget", C.c_size_t),
                 ("bvh_width", C.c_int32), ("trace_block", C.c_int32), ("trace_chunk", C.c_int32),
-                ("trace_grid", C.c_int32)]
+                ("trace_grid", C.c_int32), ("hoist_frac", C.c_float), ("leaf_max", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -62,7 +62,7 @@ class Stats(C.Structure):
 
 class SceneInfo(C.Structure):
     _fields_ = [("n_tri", C.c_int32), ("n_ref_nodes", C.c_int32), ("n_gpu_nodes", C.c_int32),
-                ("width", C.c_int32), ("max_depth", C.c_int32), ("node_bytes", C.c_size_t),
+                ("width", C.c_int32), ("max_depth", C.c_int32), ("n_hoist", C.c_int32), ("node_bytes", C.c_size_t),
                 ("tri_bytes", C.c_size_t), ("shade_bytes", C.c_size_t)]
 
 

@@ -4,22 +4,34 @@
 //
 // The reference node stores its own box plus an index to two children; the
 // kernel re-reads each child's 80-byte node just to test its box.  Here each
-// node carries its children's boxes (copied bit for bit from the reference
-// nodes, so every box test sees the same numbers), laid out per axis so one
-// 16-byte load feeds the same slab of every child:
+// node carries its children's boxes, laid out per axis so one 16-byte load
+// feeds the same slab of every child:
 //   width 2 (64 B):  [lo_x0 lo_x1 hi_x0 hi_x1] [.. y ..] [.. z ..] [c0 c1 - -]
 //   width 4 (128 B): [lo_x0..3] [hi_x0..3] [lo_y..] [hi_y..] [lo_z..] [hi_z..] [c0..c3] [-]
-// Width 4 collapses two reference levels: an interior child with the largest
+// Width 4 collapses two binary levels: an interior child with the largest
 // surface area is replaced by its own two children until four are held.
-// Nodes are numbered depth first, so a ray that descends touches nearby
-// memory.  Leaf triangles are copied into slots in the same depth-first
-// order, keeping the reference's vert_list order inside a leaf:
+// Nodes are numbered depth first; leaf triangles are copied into slots in
+// the same order, keeping vert_list order inside a leaf:
 //   tri_isect[slot] = v1 (w = triangle id bits), v2 - v1, v3 - v1
-// (the reference computes the same two differences in rayTriangleIntersection,
-// udpt.cl:328-329).  Shading data stays indexed by triangle id and is read
-// once per closest hit: normalize(vn1..3) (udpt.cl:377-379) and matID.
+// (the two differences rayTriangleIntersection computes, udpt.cl:328-329).
+// Shading data stays indexed by triangle id and is read once per closest
+// hit: normalize(vn1..3) (udpt.cl:377-379) and matID.
+//
+// Two optional transformations of the tree, neither of which changes which
+// triangle is closest along a ray (every triangle stays reachable through
+// boxes that contain it):
+//  * hoisting: triangles whose box covers a large part of the scene (walls
+//    of a room) inflate every ancestor box of their leaf, so nearly every
+//    ray walks those chains.  They are moved to a short list, each with its
+//    own box, tested by every ray before the traversal, and the tree is
+//    refitted without them;
+//  * leaf splitting: leaves of more than `leaf_max` triangles (the
+//    reference allows 10) become small subtrees split at the median
+//    centroid along their longest centroid extent.
+// With both off the node boxes are the reference's boxes bit for bit.
 #include "yn_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -35,15 +47,128 @@ inline float box_area(const yn_aabb& b)
     return dx * dy + dx * dz + dy * dz;
 }
 
+// Padded triangle box of TriangleCPU::computeAABB (TriangleCPU.cpp:51-72).
+void tri_box(const yn_triangle& t, float mn[3], float mx[3])
+{
+    const float* v1 = &t.v1.x; const float* v2 = &t.v2.x; const float* v3 = &t.v3.x;
+    for (int k = 0; k < 3; ++k) {
+        float a = v2[k] < v1[k] ? v2[k] : v1[k];
+        mn[k] = v3[k] < a ? v3[k] : a;
+        float b = v1[k] < v2[k] ? v2[k] : v1[k];
+        mx[k] = b < v3[k] ? v3[k] : b;
+        if (mx[k] - mn[k] == 0.0f) mx[k] += 0.2f;
+    }
+}
+
+float area3(const float mn[3], const float mx[3])
+{
+    float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    return dx * dy + dx * dz + dy * dz;
+}
+
+inline bool live(const yn_bvh_node& n) { return n.vert_len > 0 || n.child_idx > 0; }
+
+void set_empty(yn_bvh_node& n)
+{
+    n.vert_len = -1;
+    n.child_idx = -2;
+    for (int q = 0; q < 10; ++q) n.vert_list[q] = -1;
+}
+
+// Copy of the reference tree with hoisted triangles removed and large
+// leaves split, refitted bottom up.  Children always have larger indices
+// than their parent, in the input and in the appended nodes.
+void derive_tree(const yn_triangle* tris, const yn_bvh_node* in, int n_node, const std::vector<uint8_t>& hoisted,
+                 int leaf_max, std::vector<yn_bvh_node>& out)
+{
+    out.assign(in, in + n_node);
+    for (int j = 0; j < n_node; ++j) {
+        yn_bvh_node& nd = out[j];
+        if (!(nd.child_idx == -1 && nd.vert_len > 0)) continue;
+        int k = 0;
+        for (int q = 0; q < nd.vert_len; ++q)
+            if (!hoisted[nd.vert_list[q]]) nd.vert_list[k++] = nd.vert_list[q];
+        for (int q = k; q < 10; ++q) nd.vert_list[q] = -1;
+        nd.vert_len = k;
+        if (k == 0) set_empty(nd);
+    }
+    // split leaves (worklist; appended nodes may split again)
+    for (size_t j = 0; j < out.size(); ++j) {
+        if (!(out[j].child_idx == -1 && out[j].vert_len > leaf_max)) continue;
+        yn_bvh_node nd = out[j];
+        int n = nd.vert_len;
+        float cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float cen[10][3];
+        for (int q = 0; q < n; ++q) {
+            const yn_triangle& t = tris[nd.vert_list[q]];
+            const float* a = &t.v1.x; const float* b = &t.v2.x; const float* c = &t.v3.x;
+            for (int k = 0; k < 3; ++k) {
+                cen[q][k] = ((a[k] + b[k]) + c[k]) / 3.0f;
+                cmn[k] = std::fmin(cmn[k], cen[q][k]);
+                cmx[k] = std::fmax(cmx[k], cen[q][k]);
+            }
+        }
+        int axis = 0;
+        for (int k = 1; k < 3; ++k)
+            if (cmx[k] - cmn[k] > cmx[axis] - cmn[axis]) axis = k;
+        int ord[10];
+        for (int q = 0; q < n; ++q) ord[q] = q;
+        std::stable_sort(ord, ord + n, [&](int a, int b) { return cen[a][axis] < cen[b][axis]; });
+        yn_bvh_node a = nd, b = nd;
+        int na = n / 2;
+        for (int q = 0; q < 10; ++q) a.vert_list[q] = b.vert_list[q] = -1;
+        for (int q = 0; q < na; ++q) a.vert_list[q] = nd.vert_list[ord[q]];
+        for (int q = na; q < n; ++q) b.vert_list[q - na] = nd.vert_list[ord[q]];
+        a.vert_len = na;
+        b.vert_len = n - na;
+        a.child_idx = b.child_idx = -1;
+        int c = (int)out.size();
+        out[j].child_idx = c;
+        out[j].vert_len = -1;
+        for (int q = 0; q < 10; ++q) out[j].vert_list[q] = -1;
+        out.push_back(a);
+        out.push_back(b);
+    }
+    // refit
+    for (size_t jj = out.size(); jj-- > 0;) {
+        yn_bvh_node& nd = out[jj];
+        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        bool any = false;
+        if (nd.child_idx == -1 && nd.vert_len > 0) {
+            for (int q = 0; q < nd.vert_len; ++q) {
+                float a[3], b[3];
+                tri_box(tris[nd.vert_list[q]], a, b);
+                for (int c = 0; c < 3; ++c) { mn[c] = std::fmin(mn[c], a[c]); mx[c] = std::fmax(mx[c], b[c]); }
+            }
+            any = true;
+        } else if (nd.child_idx > 0) {
+            for (int c = nd.child_idx; c <= nd.child_idx + 1; ++c) {
+                const yn_bvh_node& ch = out[c];
+                if (!live(ch)) continue;
+                any = true;
+                const float* a = &ch.aabb.p_min.x; const float* b = &ch.aabb.p_max.x;
+                for (int k = 0; k < 3; ++k) { mn[k] = std::fmin(mn[k], a[k]); mx[k] = std::fmax(mx[k], b[k]); }
+            }
+            if (!any) set_empty(nd);
+        }
+        if (any) {
+            nd.aabb.p_min.x = mn[0]; nd.aabb.p_min.y = mn[1]; nd.aabb.p_min.z = mn[2];
+            nd.aabb.p_max.x = mx[0]; nd.aabb.p_max.y = mx[1]; nd.aabb.p_max.z = mx[2];
+        }
+    }
+}
+
 }  // namespace
 
-int make_layout(const yn_triangle* tris, int n_tri, int n_mat, const yn_bvh_node* nodes, int n_node,
-                int width, GpuLayout& L, std::string& err)
+int make_layout(const yn_triangle* tris, int n_tri, int n_mat, const yn_bvh_node* nodes_in, int n_node_in,
+                const LayoutOpts& opt, GpuLayout& L, std::string& err)
 {
     L = GpuLayout();
+    const int width = opt.width;
     if (width != 2 && width != 4) { err = "BVH width must be 2 or 4"; return -2; }
+    if (opt.leaf_max < 1) { err = "leaf_max must be >= 1"; return -2; }
     L.width = width;
-    if (n_tri < 0 || n_node < 0 || (n_tri > 0 && !tris) || (n_node > 0 && !nodes)) {
+    if (n_tri < 0 || n_node_in < 0 || (n_tri > 0 && !tris) || (n_node_in > 0 && !nodes_in)) {
         err = "bad scene arrays"; return -2;
     }
     if (n_tri > kMaxLeafFirst) { err = "too many triangles"; return -2; }
@@ -74,35 +199,78 @@ int make_layout(const yn_triangle* tris, int n_tri, int n_mat, const yn_bvh_node
         L.slot_tri.push_back(tri);
     };
 
-    if (n_node == 0) {          // brute force over every triangle (udpt.cl:280-284)
+    if (n_node_in == 0) {       // brute force over every triangle (udpt.cl:280-284)
         L.brute = 1;
         for (int i = 0; i < n_tri; ++i) add_slot(i);
         L.root_ref = kEmptyRef;
         return 0;
     }
 
-    // Classify reference nodes exactly as traverseBVH reads them.
+    // Classify nodes exactly as traverseBVH reads them; reject anything the
+    // reference would index out of bounds with.
     enum { EMPTY = 0, LEAF = 1, INNER = 2 };
-    std::vector<uint8_t> kind(n_node);
-    for (int j = 0; j < n_node; ++j) {
-        const yn_bvh_node& nd = nodes[j];
-        if (nd.child_idx == -1 && nd.vert_len > 0) {
-            if (nd.vert_len > 10) { err = "node " + std::to_string(j) + ": vert_len > 10"; return -2; }
-            for (int k = 0; k < nd.vert_len; ++k)
-                if (nd.vert_list[k] < 0 || nd.vert_list[k] >= n_tri) {
-                    err = "node " + std::to_string(j) + ": triangle index out of range"; return -2;
+    std::vector<uint8_t> kind;
+    auto classify = [&](const yn_bvh_node* nds, int n) -> int {
+        kind.assign(n, EMPTY);
+        for (int j = 0; j < n; ++j) {
+            const yn_bvh_node& nd = nds[j];
+            if (nd.child_idx == -1 && nd.vert_len > 0) {
+                if (nd.vert_len > 10) { err = "node " + std::to_string(j) + ": vert_len > 10"; return -2; }
+                for (int k = 0; k < nd.vert_len; ++k)
+                    if (nd.vert_list[k] < 0 || nd.vert_list[k] >= n_tri) {
+                        err = "node " + std::to_string(j) + ": triangle index out of range"; return -2;
+                    }
+                kind[j] = LEAF;
+            } else if (nd.child_idx > 0) {
+                if (nd.child_idx <= j || nd.child_idx + 1 >= n) {
+                    err = "node " + std::to_string(j) + ": child_idx out of range"; return -2;
                 }
-            kind[j] = LEAF;
-        } else if (nd.child_idx > 0) {
-            if (nd.child_idx <= j || nd.child_idx + 1 >= n_node) {
-                err = "node " + std::to_string(j) + ": child_idx out of range"; return -2;
+                kind[j] = INNER;
+            } else if (nd.vert_len > 0) {
+                err = "node " + std::to_string(j) + ": neither leaf nor interior"; return -2;
             }
-            kind[j] = INNER;
-        } else if (nd.vert_len > 0) {
-            err = "node " + std::to_string(j) + ": neither leaf nor interior"; return -2;
-        } else {
-            kind[j] = EMPTY;
         }
+        return 0;
+    };
+    if (int r = classify(nodes_in, n_node_in)) return r;
+
+    const yn_bvh_node* nodes = nodes_in;
+    int n_node = n_node_in;
+    std::vector<yn_bvh_node> derived;
+    std::vector<uint8_t> hoisted(n_tri, 0);
+    if (opt.hoist_frac > 0.0f && n_tri > 0 && kind[0] == INNER) {
+        const yn_aabb& rb = nodes_in[0].aabb;
+        float rmn[3] = {rb.p_min.x, rb.p_min.y, rb.p_min.z}, rmx[3] = {rb.p_max.x, rb.p_max.y, rb.p_max.z};
+        float root_a = area3(rmn, rmx);
+        std::vector<std::pair<float, int>> big;
+        for (int t = 0; t < n_tri; ++t) {
+            float a[3], b[3];
+            tri_box(tris[t], a, b);
+            float ar = area3(a, b);
+            if (ar > opt.hoist_frac * root_a) big.push_back({-ar, t});
+        }
+        std::sort(big.begin(), big.end());
+        if ((int)big.size() > kMaxHoist) big.resize(kMaxHoist);
+        std::vector<int> ids;
+        for (auto& b : big) ids.push_back(b.second);
+        std::sort(ids.begin(), ids.end());
+        for (int t : ids) {
+            hoisted[t] = 1;
+            add_slot(t);
+            float a[3], b[3];
+            tri_box(tris[t], a, b);
+            float hb[8] = {a[0], a[1], a[2], 0.0f, b[0], b[1], b[2], 0.0f};
+            L.hoist_box.insert(L.hoist_box.end(), hb, hb + 8);
+        }
+        L.n_hoist = (int)ids.size();
+    }
+    bool any_split = false;
+    for (int j = 0; j < n_node_in && !any_split; ++j) any_split = kind[j] == LEAF && nodes_in[j].vert_len > opt.leaf_max;
+    if (L.n_hoist > 0 || any_split) {
+        derive_tree(tris, nodes_in, n_node_in, hoisted, opt.leaf_max, derived);
+        nodes = derived.data();
+        n_node = (int)derived.size();
+        if (int r = classify(nodes, n_node)) return r;
     }
     std::vector<uint8_t> seen(n_node, 0);
 
@@ -119,7 +287,7 @@ int make_layout(const yn_triangle* tris, int n_tri, int n_mat, const yn_bvh_node
     if (kind[0] == EMPTY) { L.root_ref = kEmptyRef; return 0; }
 
     const int fpn = width == 2 ? 16 : 32;   // floats per node
-    // Children of a wide node: reference node ids.
+    // Children of a wide node: node ids.
     auto gather = [&](int j, int* ch) -> int {
         int n = 0;
         int c = nodes[j].child_idx;
@@ -160,6 +328,7 @@ int make_layout(const yn_triangle* tris, int n_tri, int n_mat, const yn_bvh_node
     int root = new_gnode();
     L.root_ref = (uint32_t)root;
     todo.push_back({0, root, 1});
+    std::vector<Frame> kids;
     while (!todo.empty()) {
         Frame f = todo.back();
         todo.pop_back();
@@ -167,17 +336,15 @@ int make_layout(const yn_triangle* tris, int n_tri, int n_mat, const yn_bvh_node
         int n = gather(f.ref_node, ch);
         if (n < 0) { err = "BVH is not a tree"; return -2; }
         if (f.depth * (width - 1) > L.max_depth) L.max_depth = f.depth * (width - 1);
-        float* g = &L.nodes[(size_t)f.gnode * fpn];
         uint32_t refs[4] = {kEmptyRef, kEmptyRef, kEmptyRef, kEmptyRef};
-        // First pass: leaves take slots now; interior children get node ids
-        // in order, and are expanded depth first (first child next).
-        std::vector<Frame> kids;
+        kids.clear();
         for (int k = 0; k < n; ++k) {
             int c = ch[k];
             if (seen[c] && kind[c] != EMPTY) { err = "BVH is not a tree"; return -2; }
             seen[c] = 1;
             const yn_aabb& b = nodes[c].aabb;
             float lo[3] = {b.p_min.x, b.p_min.y, b.p_min.z}, hi[3] = {b.p_max.x, b.p_max.y, b.p_max.z};
+            float* g = &L.nodes[(size_t)f.gnode * fpn];
             if (width == 2) {
                 for (int a = 0; a < 3; ++a) { g[4 * a + k] = lo[a]; g[4 * a + 2 + k] = hi[a]; }
             } else {
@@ -190,14 +357,11 @@ int make_layout(const yn_triangle* tris, int n_tri, int n_mat, const yn_bvh_node
                 refs[k] = kLeafBit | (first << 4) | (uint32_t)nodes[c].vert_len;
             } else if (kind[c] == INNER) {
                 int id = new_gnode();
-                g = &L.nodes[(size_t)f.gnode * fpn];   // storage may have moved
                 refs[k] = (uint32_t)id;
                 kids.push_back({c, id, f.depth + 1});
-            } else {
-                refs[k] = kEmptyRef;
             }
         }
-        int* gi = reinterpret_cast<int*>(g);
+        int* gi = reinterpret_cast<int*>(&L.nodes[(size_t)f.gnode * fpn]);
         if (width == 2) { gi[12] = (int)refs[0]; gi[13] = (int)refs[1]; gi[14] = 0; gi[15] = 0; }
         else { for (int k = 0; k < 4; ++k) gi[24 + k] = (int)refs[k]; }
         for (size_t k = kids.size(); k-- > 0;) todo.push_back(kids[k]);

@@ -206,7 +206,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
             }
             f4 so = hp + wi * kEps;
             float tq;
-            if (cap_on && !(st & F_CAPSH)) {
+            if (cap_on && (st & F_DIRECT) && !(st & F_CAPSH)) {
                 cap.shadow_o[pix] = make_float4(so.x, so.y, so.z, len);
                 cap.shadow_d[pix] = make_float4(wi.x, wi.y, wi.z, 1.0f);
             }
@@ -225,7 +225,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
                 nprob = P.nee_prob[p];
                 code = ST_DONE;     // consumed
             }
-            if (cap_on && !(st & F_CAPSH)) { cap.shadow_occ[pix] = occ; st |= F_CAPSH; }
+            if (cap_on && (st & F_DIRECT) && !(st & F_CAPSH)) { cap.shadow_occ[pix] = occ; st |= F_CAPSH; }
             Mat m = load_mat(S.mats, S.tri_mat[tri]);
             em = m.ke;
             f4 w_o = -rd;

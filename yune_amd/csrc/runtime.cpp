@@ -37,7 +37,7 @@ namespace {
 struct SceneRec {
     bool live = false;
     DevScene dev{};
-    void* bufs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* bufs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     yn_scene_info info{};
 };
 
@@ -61,6 +61,7 @@ struct JobRec {
 };
 
 constexpr int kMaxScenes = 64;
+constexpr int kMaxStack = 256;      // traversal stack entries a scene may need
 constexpr int kMaxJobs = 4096;
 constexpr int kEvPool = 4096;
 
@@ -92,6 +93,11 @@ struct yn_ctx {
     int ev_used = 0;
     int pt_job_slot = -1;             // job slot used by yn_pathtracer
     size_t job_bytes = 0;
+    // traversal stack overflow area (entries beyond the LDS part)
+    uint32_t* ovf_ref = nullptr;
+    uint16_t* ovf_t = nullptr;
+    int ovf_depth = 0;
+    int64_t ovf_threads = 0;
 };
 
 namespace {
@@ -121,6 +127,27 @@ int dalloc(yn_ctx* c, T** p, size_t count)
         return e == hipErrorOutOfMemory ? YN_ERR_NO_MEMORY : YN_ERR_HIP;
     }
     *p = static_cast<T*>(q);
+    return 0;
+}
+
+// Overflow area for traversal stacks deeper than the LDS part: `depth`
+// entries per thread of a grid of `threads`.
+int ensure_overflow(yn_ctx* c, int depth, int64_t threads)
+{
+    depth = std::max(0, depth);
+    if (depth <= c->ovf_depth && threads <= c->ovf_threads && c->ovf_ref) return 0;
+    depth = std::max(depth, c->ovf_depth);
+    threads = std::max(threads, c->ovf_threads);
+    if (c->ovf_ref) { hipFree(c->ovf_ref); hipFree(c->ovf_t); c->ovf_ref = nullptr; c->ovf_t = nullptr; }
+    c->ovf_depth = 0;
+    c->ovf_threads = 0;
+    size_t n = (size_t)std::max(depth, 1) * (size_t)threads;
+    int r = dalloc(c, &c->ovf_ref, n);
+    if (r) return r;
+    r = dalloc(c, &c->ovf_t, n);
+    if (r) { hipFree(c->ovf_ref); c->ovf_ref = nullptr; return r; }
+    c->ovf_depth = depth;
+    c->ovf_threads = threads;
     return 0;
 }
 
@@ -243,6 +270,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     int block = c->cfg.trace_block;
     int grid = c->cfg.trace_grid > 0 ? c->cfg.trace_grid : c->n_cu * (2048 / block);
     int shade_grid = c->n_cu * 8;
+    if (int r = ensure_overflow(c, max_depth - trace_lds_stack(), (int64_t)grid * block)) return r;
     bool count = stat_job.counting != 0;
 
     int ev_base = 0;
@@ -264,6 +292,9 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             ta.single_scene = single;
             ta.chunk = c->cfg.trace_chunk;
             ta.ctr = c->ctr;
+            ta.ovf_ref = c->ovf_ref;
+            ta.ovf_t = c->ovf_t;
+            ta.ovf_depth = c->ovf_depth;
             // extension rays
             ta.q = c->E[cur];
             ta.hit_out = c->P.hit;
@@ -443,6 +474,9 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.bvh_width == 0) k.bvh_width = 4;
     if (k.trace_block == 0) k.trace_block = 128;
     if (k.trace_chunk <= 0) k.trace_chunk = 64;
+    if (k.hoist_frac == 0.0f) k.hoist_frac = 0.01f;
+    if (k.leaf_max <= 0) k.leaf_max = 4;
+    if (k.leaf_max > 10) k.leaf_max = 10;
     if (k.bvh_width != 2 && k.bvh_width != 4) return fail(nullptr, YN_ERR_INVALID, "bvh_width must be 2 or 4");
     if (k.trace_block != 64 && k.trace_block != 128 && k.trace_block != 256)
         return fail(nullptr, YN_ERR_INVALID, "trace_block must be 64, 128 or 256");
@@ -489,6 +523,8 @@ void yn_destroy(yn_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     free_all(c);
     if (c->d_segs) hipFree(c->d_segs);
+    if (c->ovf_ref) hipFree(c->ovf_ref);
+    if (c->ovf_t) hipFree(c->ovf_t);
     if (c->h_ctr) hipHostFree(c->h_ctr);
     for (int i = 0; i < kEvPool; ++i) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -514,9 +550,13 @@ int yn_scene_create(yn_ctx* c, const yn_triangle* tris, int32_t n_tri, const yn_
     if (sid < 0) return fail(c, YN_ERR_NO_MEMORY, "too many scenes");
     GpuLayout L;
     std::string err;
-    int r = make_layout(tris, n_tri, n_mat, nodes, n_node, c->cfg.bvh_width, L, err);
+    LayoutOpts lo;
+    lo.width = c->cfg.bvh_width;
+    lo.hoist_frac = c->cfg.hoist_frac;
+    lo.leaf_max = c->cfg.leaf_max;
+    int r = make_layout(tris, n_tri, n_mat, nodes, n_node, lo, L, err);
     if (r) return fail(c, r, err);
-    if (L.max_depth > 96) return fail(c, YN_ERR_INVALID, "BVH deeper than the traversal stack (96)");
+    if (L.max_depth > kMaxStack) return fail(c, YN_ERR_INVALID, "BVH deeper than the traversal stack bound");
     SceneRec& S = c->scenes[sid];
     S = SceneRec();
     auto up = [&](int k, const void* src, size_t bytes) -> int {
@@ -528,7 +568,8 @@ int yn_scene_create(yn_ctx* c, const yn_triangle* tris, int32_t n_tri, const yn_
     };
     if ((r = up(0, L.nodes.data(), L.nodes.size() * 4)) || (r = up(1, L.tri_isect.data(), L.tri_isect.size() * 4)) ||
         (r = up(2, L.tri_shade.data(), L.tri_shade.size() * 4)) || (r = up(3, L.tri_mat.data(), L.tri_mat.size() * 4)) ||
-        (r = up(4, mats, sizeof(yn_material) * (size_t)n_mat))) {
+        (r = up(4, mats, sizeof(yn_material) * (size_t)n_mat)) ||
+        (r = up(5, L.hoist_box.data(), L.hoist_box.size() * 4))) {
         for (void*& b : S.bufs) { if (b) hipFree(b); b = nullptr; }
         return r;
     }
@@ -538,6 +579,7 @@ int yn_scene_create(yn_ctx* c, const yn_triangle* tris, int32_t n_tri, const yn_
     D.tri_shade = (const float4*)S.bufs[2];
     D.tri_mat = (const int32_t*)S.bufs[3];
     D.mats = (const yn_material*)S.bufs[4];
+    D.hoist_box = (const float4*)S.bufs[5];
     D.light = *light;
     // the reference constant has these w's (udpt.cl:97-104)
     D.light.pos.w = 1.0f; D.light.normal.w = 0.0f; D.light.ke.w = 0.0f;
@@ -547,12 +589,14 @@ int yn_scene_create(yn_ctx* c, const yn_triangle* tris, int32_t n_tri, const yn_
     D.width = L.width;
     D.brute = L.brute;
     D.n_slots = (int32_t)L.slot_tri.size();
+    D.n_hoist = L.n_hoist;
     HIPCK(c, hipMemcpy(c->d_scenes + sid, &D, sizeof(DevScene), hipMemcpyHostToDevice));
     S.info.n_tri = n_tri;
     S.info.n_ref_nodes = n_node;
     S.info.n_gpu_nodes = L.n_nodes;
     S.info.width = L.width;
     S.info.max_depth = L.max_depth;
+    S.info.n_hoist = L.n_hoist;
     S.info.node_bytes = L.nodes.size() * 4;
     S.info.tri_bytes = L.tri_isect.size() * 4;
     S.info.shade_bytes = L.tri_shade.size() * 4 + L.tri_mat.size() * 4;
@@ -840,6 +884,11 @@ int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri,
         ta.occ_out = any ? c->P.occ : nullptr;
         ta.work = c->ctr->q + 4;
         int grid = c->n_cu * (2048 / c->cfg.trace_block);
+        if (int rr = ensure_overflow(c, S.info.max_depth - trace_lds_stack(), (int64_t)grid * c->cfg.trace_block))
+            return rr;
+        ta.ovf_ref = c->ovf_ref;
+        ta.ovf_t = c->ovf_t;
+        ta.ovf_depth = c->ovf_depth;
         if (launch_trace(ta, any != 0, S.info.width, S.info.max_depth, c->cfg.trace_block, grid, false, s) != 0)
             return fail(c, YN_ERR_INVALID, "BVH too deep");
         HIPCK(c, hipGetLastError());

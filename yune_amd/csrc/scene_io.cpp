@@ -248,7 +248,7 @@ void push_box(std::vector<yn_triangle>& T, float cx, float y0, float cz, float s
 // Oren-Nayar sigma 0; a 0.5 x 0.5 quad light 0.01 under the ceiling
 // emitting (17,12,4) downwards, whose two triangles sit 0.005 above the
 // analytic quad; a white diffuse box and a modified-Phong box with
-// px + py = 50.
+// px + py = 50, standing 0.001 above the floor.
 void gen_cornell(HostScene& S)
 {
     S = HostScene();
@@ -295,8 +295,9 @@ void gen_cornell(HostScene& S)
               d[3] = {lx0, py, lz0 + ls}, n[3] = {0, -1, 0};
         push_quad(T, a, b, c, d, n, 3);               // light panel triangles
     }
-    push_box(T, 0.35f, -1.0f, -2.85f, 0.6f, 0.6f, 0.6f, -17.0f, 0);   // short white box
-    push_box(T, -0.35f, -1.0f, -3.45f, 0.6f, 1.2f, 0.6f, 20.0f, 4);   // tall glossy box
+    // boxes stand 0.001 above the floor so no face is coplanar with it
+    push_box(T, 0.35f, -0.999f, -2.85f, 0.6f, 0.6f, 0.6f, -17.0f, 0);   // short white box
+    push_box(T, -0.35f, -0.999f, -3.45f, 0.6f, 1.2f, 0.6f, 20.0f, 4);   // tall glossy box
     S.light.pos = {lx0, ly, lz0, 1};
     S.light.normal = {0, -1, 0, 0};
     S.light.ke = {17, 12, 4, 0};

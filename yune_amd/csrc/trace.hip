@@ -12,18 +12,20 @@
 //    shrinks and ignores t; this traversal visits children near-first and
 //    skips a box whose entry distance is beyond the best hit so far (with a
 //    1e-4 relative margin, so rounding in the box entry can never hide a
-//    triangle the reference would find).  Boxes, slab test and
-//    Moller-Trumbore use the reference's operands and order, so t, u, v and
-//    the accept/reject decisions are bit-identical to the CPU restatement.
-//    Only equal-t ties may resolve to a different triangle (test order).
+//    triangle the reference would find).  The slab test and Moller-Trumbore
+//    use the reference's operands and order, so t, u, v and accept/reject
+//    decisions are bit-identical to the CPU restatement.  Only equal-t ties
+//    may resolve to a different triangle (test order).
 //  * any hit (shadow rays, udpt.cl:307-308): true if some triangle has t in
 //    (0, tmax).
 //
-// Layout: per node the children's boxes are stored per axis (64 B for two
-// children, 128 B for four), child references carry leaf ranges inline.
-// Each lane owns one ray; its traversal stack lives in LDS, interleaved by
-// lane so the 64 lanes of a wave hit 64 different banks.  Waves pull chunks
-// of 64 rays from the queue with one atomic per chunk.
+// Each lane owns one ray.  Its traversal stack keeps the top LST entries in
+// LDS, interleaved by lane so a wave's 64 lanes hit 64 banks, and spills
+// deeper entries to a per-thread global area (rarely touched: stack depth
+// stays far below the tree depth bound).  An entry is a child reference and
+// a 16-bit lower bound of its entry distance, so a deferred child that now
+// lies beyond the best hit is dropped on pop without fetching it.  Waves pull
+// chunks of rays from the queue with one atomic per chunk.
 #include "yn_device.h"
 #include "yn_internal_dev.h"
 
@@ -58,7 +60,6 @@ struct RayState {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
     float best;             // current tmax
     float cull;             // best * (1 + 1e-4)
-    int slot;               // slot of the best hit, -1 none
     int tri;
     float u, v;
     int overflow;           // traversal stack would have overflowed
@@ -66,7 +67,6 @@ struct RayState {
 
 // rayTriangleIntersection (udpt.cl:326-390) on a stored slot; xyz only
 // (the w terms of the reference's dots are exact zeros).
-template <bool ANY>
 __device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int s, RayState& r)
 {
     float4 A = tri[3 * s], B = tri[3 * s + 1], C = tri[3 * s + 2];
@@ -89,7 +89,6 @@ __device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int s, 
     if (t > 0.0f && t < r.best) {
         r.best = t;
         r.cull = t * 1.0001f;
-        r.slot = s;
         r.tri = __float_as_int(A.w);
         r.u = u;
         r.v = v;
@@ -108,38 +107,90 @@ __device__ __forceinline__ bool leaf_test(const float4* __restrict__ tri, uint32
     bool got = false;
     for (int k = 0; k < cnt; ++k) {
         if (COUNT) c.tris++;
-        got |= tri_test<ANY>(tri, first + k, r);
+        got |= tri_test(tri, first + k, r);
         if (ANY && got) return true;
     }
     return got;
 }
 
+// Per-lane traversal stack: LDS for the top LST entries, global beyond.
+template <int LST, int BLOCK>
+struct Stack {
+    uint32_t* lref;         // LDS, this lane's column
+    uint16_t* lt;
+    uint32_t* gref;         // global overflow, this thread's column
+    uint16_t* gt;
+    uint32_t gstride;       // threads in the grid
+    int gcap;               // overflow entries available
+    int sp;
+
+    __device__ __forceinline__ bool push(uint32_t ref, float t)
+    {
+        uint16_t tb = (uint16_t)(__float_as_uint(t) >> 16);   // truncation: a lower bound of t >= 0
+        if (sp < LST) {
+            lref[sp * BLOCK] = ref;
+            lt[sp * BLOCK] = tb;
+        } else {
+            int k = sp - LST;
+            if (k >= gcap) return false;
+            gref[(size_t)k * gstride] = ref;
+            gt[(size_t)k * gstride] = tb;
+        }
+        ++sp;
+        return true;
+    }
+    __device__ __forceinline__ uint32_t pop(float& t)
+    {
+        --sp;
+        uint32_t ref;
+        uint16_t tb;
+        if (sp < LST) {
+            ref = lref[sp * BLOCK];
+            tb = lt[sp * BLOCK];
+        } else {
+            size_t k = (size_t)(sp - LST) * gstride;
+            ref = gref[k];
+            tb = gt[k];
+        }
+        t = __uint_as_float((uint32_t)tb << 16);
+        return ref;
+    }
+};
+
 // Traverse one ray; returns true if a hit was found.
-template <int W, bool ANY, int STACK, int BLOCK, bool COUNT>
-__device__ bool traverse(const DevScene& S, RayState& r, uint32_t* stk, Counts& c)
+template <int W, bool ANY, int LST, int BLOCK, bool COUNT>
+__device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, Counts& c)
 {
     const float4* __restrict__ nodes = S.nodes;
     const float4* __restrict__ tri = S.tri_isect;
+    bool got = false;
     if (S.brute) {
-        bool got = false;
         for (int s = 0; s < S.n_slots; ++s) {
             if (COUNT) c.tris++;
-            got |= tri_test<ANY>(tri, s, r);
+            got |= tri_test(tri, s, r);
             if (ANY && got) return true;
         }
         return got;
     }
+    // oversized triangles hoisted out of the tree, each behind its own box
+    for (int s = 0; s < S.n_hoist; ++s) {
+        float4 lo = S.hoist_box[2 * s], hi = S.hoist_box[2 * s + 1];
+        float e;
+        if (!box_hit(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e) || e > r.cull)
+            continue;
+        if (COUNT) c.tris++;
+        got |= tri_test(tri, s, r);
+        if (ANY && got) return true;
+    }
     uint32_t node = S.root_ref;
-    if (node == kEmptyRef) return false;
+    if (node == kEmptyRef) return got;
     if (node & kLeafBit) {
         float e;
         if (!box_hit(S.root_box[0], S.root_box[1], S.root_box[2], S.root_box[3], S.root_box[4], S.root_box[5],
                      r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e))
-            return false;
-        return leaf_test<ANY, COUNT>(tri, node, r, c);
+            return got;
+        return leaf_test<ANY, COUNT>(tri, node, r, c) || got;
     }
-    int sp = 0;
-    bool got = false;
     for (;;) {
         if (!(node & kLeafBit)) {
             if (COUNT) c.nodes++;
@@ -154,10 +205,9 @@ __device__ bool traverse(const DevScene& S, RayState& r, uint32_t* stk, Counts& 
                 h1 = h1 && e1 <= r.cull && (uint32_t)C.y != kEmptyRef;
                 if (h0 && h1) {
                     uint32_t nearc = (uint32_t)C.x, farc = (uint32_t)C.y;
-                    if (e1 < e0) { nearc = (uint32_t)C.y; farc = (uint32_t)C.x; }
-                    if (sp >= STACK) { r.overflow = 1; return got; }   // cannot happen: depth <= STACK
-                    stk[sp * BLOCK] = farc;
-                    ++sp;
+                    float fe = e1;
+                    if (e1 < e0) { nearc = (uint32_t)C.y; farc = (uint32_t)C.x; fe = e0; }
+                    if (!st.push(farc, fe)) { r.overflow = 1; return got; }   // cannot happen: depth bound
                     node = nearc;
                     continue;
                 } else if (h0) { node = (uint32_t)C.x; continue; }
@@ -192,23 +242,23 @@ __device__ bool traverse(const DevScene& S, RayState& r, uint32_t* stk, Counts& 
                         key[b + 1] = kk;
                         ch[b + 1] = cc;
                     }
-                    if (sp + nh - 1 > STACK) { r.overflow = 1; return got; }   // cannot happen: depth <= STACK
-                    for (int a = 0; a < nh - 1; ++a) { stk[sp * BLOCK] = ch[a]; ++sp; }
+                    for (int a = 0; a < nh - 1; ++a)
+                        if (!st.push(ch[a], key[a])) { r.overflow = 1; return got; }   // cannot happen
                     node = ch[nh - 1];
                     continue;
                 }
             }
         } else {
-            bool g = leaf_test<ANY, COUNT>(tri, node, r, c);
-            got |= g;
+            got |= leaf_test<ANY, COUNT>(tri, node, r, c);
             if (ANY && got) return true;
         }
-        // pop
+        // pop; a deferred child whose entry now lies beyond the best hit is dropped
         for (;;) {
-            if (sp == 0) return got;
-            --sp;
-            node = stk[sp * BLOCK];
-            // a deferred child may now lie beyond the best hit: skip it
+            if (st.sp == 0) return got;
+            float et;
+            uint32_t ref = st.pop(et);
+            if (et > r.cull) continue;
+            node = ref;
             break;
         }
     }
@@ -216,11 +266,19 @@ __device__ bool traverse(const DevScene& S, RayState& r, uint32_t* stk, Counts& 
 
 }  // namespace
 
-template <int W, bool ANY, int STACK, int BLOCK, bool COUNT>
+template <int W, bool ANY, int LST, int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a)
 {
-    __shared__ uint32_t stk_all[STACK * BLOCK];
-    uint32_t* stk = stk_all + threadIdx.x;
+    __shared__ uint32_t stk_all[LST * BLOCK];
+    __shared__ uint16_t stkt_all[LST * BLOCK];
+    Stack<LST, BLOCK> st;
+    st.lref = stk_all + threadIdx.x;
+    st.lt = stkt_all + threadIdx.x;
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    st.gstride = gridDim.x * BLOCK;
+    st.gref = a.ovf_ref + gtid;
+    st.gt = a.ovf_t + gtid;
+    st.gcap = a.ovf_depth;
     const int lane = (int)__lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.zero0) { *a.zero0 = 0; if (a.zero1) *a.zero1 = 0; }
     const uint32_t n = *a.q.count;
@@ -245,8 +303,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a)
             r.ix = 1 / r.dx; r.iy = 1 / r.dy; r.iz = 1 / r.dz;   // dir_inv (udpt.cl:395)
             r.best = ot.w;
             r.cull = ot.w * 1.0001f;
-            r.slot = -1; r.tri = -1; r.u = 0.0f; r.v = 0.0f; r.overflow = 0;
-            bool got = traverse<W, ANY, STACK, BLOCK, COUNT>(S, r, stk, c);
+            r.tri = -1; r.u = 0.0f; r.v = 0.0f; r.overflow = 0;
+            st.sp = 0;
+            bool got = traverse<W, ANY, LST, BLOCK, COUNT>(S, r, st, c);
             ++nrays;
             if (r.overflow) atomicOr(&a.ctr->q[7], 1u);
             if (ANY) {
@@ -269,43 +328,42 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a)
     }
 }
 
-// Host-side launcher: picks the instantiation for width / stack depth.
-#define YN_TRACE_CASE(W, ANY, ST, BL, CNT)                                                        \
-    hipLaunchKernelGGL((k_trace<W, ANY, ST, BL, CNT>), dim3(grid), dim3(BL), 0, stream, a)
+namespace {
+constexpr int kLdsStack = 16;
 
 template <int W, bool ANY, int BL>
-static int launch_stack(const TraceArgs& a, int stack, int grid, bool count, hipStream_t stream)
+void launch_b(const TraceArgs& a, int grid, bool count, hipStream_t s)
 {
-    if (stack <= 32) {
-        if (count) YN_TRACE_CASE(W, ANY, 32, BL, true); else YN_TRACE_CASE(W, ANY, 32, BL, false);
-    } else if (stack <= 64) {
-        if (count) YN_TRACE_CASE(W, ANY, 64, BL, true); else YN_TRACE_CASE(W, ANY, 64, BL, false);
-    } else if (stack <= 96) {
-        if (count) YN_TRACE_CASE(W, ANY, 96, BL, true); else YN_TRACE_CASE(W, ANY, 96, BL, false);
-    } else {
-        return -1;
-    }
-    return 0;
+    if (count) hipLaunchKernelGGL((k_trace<W, ANY, kLdsStack, BL, true>), dim3(grid), dim3(BL), 0, s, a);
+    else hipLaunchKernelGGL((k_trace<W, ANY, kLdsStack, BL, false>), dim3(grid), dim3(BL), 0, s, a);
 }
 
 template <bool ANY>
-static int launch_w(const TraceArgs& a, int width, int stack, int block, int grid, bool count, hipStream_t s)
+void launch_w(const TraceArgs& a, int width, int block, int grid, bool count, hipStream_t s)
 {
     if (width == 2) {
-        if (block == 64) return launch_stack<2, ANY, 64>(a, stack, grid, count, s);
-        if (block == 128) return launch_stack<2, ANY, 128>(a, stack, grid, count, s);
-        return launch_stack<2, ANY, 256>(a, stack, grid, count, s);
+        if (block == 64) launch_b<2, ANY, 64>(a, grid, count, s);
+        else if (block == 128) launch_b<2, ANY, 128>(a, grid, count, s);
+        else launch_b<2, ANY, 256>(a, grid, count, s);
+    } else {
+        if (block == 64) launch_b<4, ANY, 64>(a, grid, count, s);
+        else if (block == 128) launch_b<4, ANY, 128>(a, grid, count, s);
+        else launch_b<4, ANY, 256>(a, grid, count, s);
     }
-    if (block == 64) return launch_stack<4, ANY, 64>(a, stack, grid, count, s);
-    if (block == 128) return launch_stack<4, ANY, 128>(a, stack, grid, count, s);
-    return launch_stack<4, ANY, 256>(a, stack, grid, count, s);
 }
+}  // namespace
 
+int trace_lds_stack() { return kLdsStack; }
+
+// `stack`: traversal stack entries the scene can need; the launch is refused
+// when LDS plus the overflow area cannot hold them.
 int launch_trace(const TraceArgs& a, bool any_hit, int width, int stack, int block, int grid, bool count,
                  hipStream_t stream)
 {
-    return any_hit ? launch_w<true>(a, width, stack, block, grid, count, stream)
-                   : launch_w<false>(a, width, stack, block, grid, count, stream);
+    if (stack > kLdsStack + a.ovf_depth) return -1;
+    if (any_hit) launch_w<true>(a, width, block, grid, count, stream);
+    else launch_w<false>(a, width, block, grid, count, stream);
+    return 0;
 }
 
 }  // namespace yune

@@ -20,13 +20,14 @@ struct DevScene {
     const float4* tri_shade;    // 3 float4 per triangle: normalised vertex normals
     const int32_t* tri_mat;     // matID per triangle
     const yn_material* mats;
+    const float4* hoist_box;    // 2 float4 per hoisted slot: lo, hi
     yn_light_quad light;
     uint32_t root_ref;
     float root_box[6];
     int32_t width;              // 2 or 4
     int32_t brute;              // 1: no BVH, test every slot in order
     int32_t n_slots;
-    int32_t pad;
+    int32_t n_hoist;            // slots [0, n_hoist) are tested before traversal
 };
 
 // Per-job constants (a job = one image being accumulated).

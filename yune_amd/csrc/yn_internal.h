@@ -32,6 +32,8 @@ struct GpuLayout {
     float root_box[6] = {0, 0, 0, 0, 0, 0};   // tested only when the root is a leaf
     int max_depth = 0;                  // stack entries traversal can need
     int brute = 0;                      // no BVH: every triangle, in index order
+    int n_hoist = 0;                    // slots [0, n_hoist): tested by every ray first
+    std::vector<float> hoist_box;       // per hoisted slot: lo.xyz, -, hi.xyz, -
     std::vector<float> tri_isect;       // per slot: v1 (w = id bits), e1, e2 -> 12 floats
     std::vector<float> tri_shade;       // per triangle: normalize(vn1..3) -> 12 floats
     std::vector<int32_t> tri_mat;       // per triangle: matID
@@ -40,8 +42,18 @@ struct GpuLayout {
 
 // Validates the reference arrays (every index in range, leaves 1..10) and
 // builds the GPU layout.  Returns 0 or a negative status with `err` set.
+// hoist_frac > 0: triangles whose box area exceeds hoist_frac x the root's
+// (at most kMaxHoist, largest first) leave their leaves and are tested by
+// every ray before traversal; the tree is refitted without them.
+// leaf_max < 10: larger leaves are split into subtrees.
+constexpr int kMaxHoist = 64;
+struct LayoutOpts {
+    int width = 4;
+    float hoist_frac = 0.01f;
+    int leaf_max = 10;
+};
 int make_layout(const yn_triangle* tris, int n_tri, int n_mat, const yn_bvh_node* nodes,
-                int n_node, int width, GpuLayout& out, std::string& err);
+                int n_node, const LayoutOpts& opt, GpuLayout& out, std::string& err);
 
 // ---- scenes (scene_io.cpp) ----------------------------------------------
 struct HostScene {

@@ -21,6 +21,9 @@ struct TraceArgs {
     uint32_t* zero0;            // counters this launch clears (next queues)
     uint32_t* zero1;
     DevCounters* ctr;
+    uint32_t* ovf_ref;          // traversal stack overflow area: ovf_depth entries
+    uint16_t* ovf_t;            //   per grid thread, strided by the grid size
+    int32_t ovf_depth;
 };
 
 // Segment of primary rays: a rectangle of one job's image at one frame.
@@ -96,6 +99,7 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
 
 int launch_trace(const TraceArgs& a, bool any_hit, int width, int stack, int block, int grid, bool count,
                  hipStream_t stream);
+int trace_lds_stack();          // traversal stack entries kept in LDS
 void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t stream);
 void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, hipStream_t stream);
 void launch_resolve(const ResolveArgs& a, hipStream_t stream);
