@@ -472,7 +472,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.max_paths <= 0) k.max_paths = 1 << 23;
     if (k.mem_budget == 0) k.mem_budget = (size_t)64 << 30;
     if (k.bvh_width == 0) k.bvh_width = 4;
-    if (k.trace_block == 0) k.trace_block = 128;
+    if (k.trace_block == 0) k.trace_block = 256;
     if (k.trace_chunk <= 0) k.trace_chunk = 64;
     if (k.hoist_frac == 0.0f) k.hoist_frac = 0.01f;
     if (k.leaf_max <= 0) k.leaf_max = 4;

@@ -33,25 +33,32 @@ namespace yune {
 
 namespace {
 
+// One slab of rayAabbIntersection (udpt.cl:397-418).  min_diff can only be
+// NaN when dir_inv is infinite (a zero or denormal direction component) and
+// the plane passes through the origin; EXACT keeps the reference's NaN skip,
+// the other form is used for waves whose rays all have finite dir_inv, where
+// the two agree bit for bit.
+template <bool EXACT>
 __device__ __forceinline__ void slab(float lo, float hi, float o, float inv, float& tmin, float& tmax)
 {
     float a = (lo - o) * inv;           // min_diff (udpt.cl:397)
     float b = (hi - o) * inv;           // max_diff (udpt.cl:398)
-    if (!(a != a)) {                    // !isnan(min_diff)
+    if (!EXACT || !(a != a)) {          // !isnan(min_diff)
         tmin = fmaxf(fminf(a, b), tmin);
         tmax = fminf(fmaxf(a, b), tmax);
     }
 }
 
 // rayAabbIntersection (udpt.cl:392-431) + entry distance for culling.
+template <bool EXACT = true>
 __device__ __forceinline__ bool box_hit(float lox, float loy, float loz, float hix, float hiy, float hiz,
                                         float ox, float oy, float oz, float ix, float iy, float iz,
                                         float& enter)
 {
     float tmin = -INFINITY, tmax = INFINITY;
-    slab(lox, hix, ox, ix, tmin, tmax);
-    slab(loy, hiy, oy, iy, tmin, tmax);
-    slab(loz, hiz, oz, iz, tmin, tmax);
+    slab<EXACT>(lox, hix, ox, ix, tmin, tmax);
+    slab<EXACT>(loy, hiy, oy, iy, tmin, tmax);
+    slab<EXACT>(loz, hiz, oz, iz, tmin, tmax);
     enter = fmaxf(tmin, 0.0f);
     return tmax > enter;
 }
@@ -158,7 +165,7 @@ struct Stack {
 };
 
 // Traverse one ray; returns true if a hit was found.
-template <int W, bool ANY, int LST, int BLOCK, bool COUNT>
+template <int W, bool ANY, int LST, int BLOCK, bool COUNT, bool EXACT>
 __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, Counts& c)
 {
     const float4* __restrict__ nodes = S.nodes;
@@ -199,8 +206,8 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
                 float4 X = n[0], Y = n[1], Z = n[2];
                 int4 C = *reinterpret_cast<const int4*>(n + 3);
                 float e0, e1;
-                bool h0 = box_hit(X.x, Y.x, Z.x, X.z, Y.z, Z.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e0);
-                bool h1 = box_hit(X.y, Y.y, Z.y, X.w, Y.w, Z.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e1);
+                bool h0 = box_hit<EXACT>(X.x, Y.x, Z.x, X.z, Y.z, Z.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e0);
+                bool h1 = box_hit<EXACT>(X.y, Y.y, Z.y, X.w, Y.w, Z.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e1);
                 h0 = h0 && e0 <= r.cull && (uint32_t)C.x != kEmptyRef;
                 h1 = h1 && e1 <= r.cull && (uint32_t)C.y != kEmptyRef;
                 if (h0 && h1) {
@@ -216,35 +223,37 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
                 const float4* n = nodes + 8 * (size_t)node;
                 float4 LX = n[0], HX = n[1], LY = n[2], HY = n[3], LZ = n[4], HZ = n[5];
                 int4 C = *reinterpret_cast<const int4*>(n + 6);
-                float e[4];
-                bool h[4];
-                h[0] = box_hit(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e[0]);
-                h[1] = box_hit(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e[1]);
-                h[2] = box_hit(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e[2]);
-                h[3] = box_hit(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e[3]);
-                uint32_t cr[4] = {(uint32_t)C.x, (uint32_t)C.y, (uint32_t)C.z, (uint32_t)C.w};
-                // order hit children by entry distance, far ones pushed first
-                float key[4];
-                uint32_t ch[4];
-                int nh = 0;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    bool ok = h[k] && e[k] <= r.cull && cr[k] != kEmptyRef;
-                    if (ok) { key[nh] = e[k]; ch[nh] = cr[k]; ++nh; }
-                }
-                if (nh > 0) {
-                    // insertion sort, descending entry distance
-                    for (int a = 1; a < nh; ++a) {
-                        float kk = key[a];
-                        uint32_t cc = ch[a];
-                        int b = a - 1;
-                        while (b >= 0 && key[b] < kk) { key[b + 1] = key[b]; ch[b + 1] = ch[b]; --b; }
-                        key[b + 1] = kk;
-                        ch[b + 1] = cc;
-                    }
-                    for (int a = 0; a < nh - 1; ++a)
-                        if (!st.push(ch[a], key[a])) { r.overflow = 1; return got; }   // cannot happen
-                    node = ch[nh - 1];
+                float k0, k1, k2, k3;
+                bool h0 = box_hit<EXACT>(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k0);
+                bool h1 = box_hit<EXACT>(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k1);
+                bool h2 = box_hit<EXACT>(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k2);
+                bool h3 = box_hit<EXACT>(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k3);
+                uint32_t c0 = (uint32_t)C.x, c1 = (uint32_t)C.y, c2 = (uint32_t)C.z, c3 = (uint32_t)C.w;
+                // missed, culled or absent children sort last (key +inf)
+                k0 = (h0 && k0 <= r.cull && c0 != kEmptyRef) ? k0 : INFINITY;
+                k1 = (h1 && k1 <= r.cull && c1 != kEmptyRef) ? k1 : INFINITY;
+                k2 = (h2 && k2 <= r.cull && c2 != kEmptyRef) ? k2 : INFINITY;
+                k3 = (h3 && k3 <= r.cull && c3 != kEmptyRef) ? k3 : INFINITY;
+                // 4-input sorting network, ascending entry distance
+#define YN_CS(ka, ca, kb, cb)                                              \
+    {                                                                      \
+        bool sw = kb < ka;                                                 \
+        float tk = sw ? kb : ka; kb = sw ? ka : kb; ka = tk;               \
+        uint32_t tc = sw ? cb : ca; cb = sw ? ca : cb; ca = tc;            \
+    }
+                YN_CS(k0, c0, k1, c1);
+                YN_CS(k2, c2, k3, c3);
+                YN_CS(k0, c0, k2, c2);
+                YN_CS(k1, c1, k3, c3);
+                YN_CS(k1, c1, k2, c2);
+#undef YN_CS
+                if (k0 != INFINITY) {
+                    bool ok = true;
+                    if (k3 != INFINITY) ok &= st.push(c3, k3);
+                    if (k2 != INFINITY) ok &= st.push(c2, k2);
+                    if (k1 != INFINITY) ok &= st.push(c1, k1);
+                    if (!ok) { r.overflow = 1; return got; }   // cannot happen: depth bound
+                    node = c0;
                     continue;
                 }
             }
@@ -305,7 +314,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a)
             r.cull = ot.w * 1.0001f;
             r.tri = -1; r.u = 0.0f; r.v = 0.0f; r.overflow = 0;
             st.sp = 0;
-            bool got = traverse<W, ANY, LST, BLOCK, COUNT>(S, r, st, c);
+            // only an infinite dir_inv (a zero or denormal direction
+            // component) can make the reference's NaN guard matter
+            bool inf_inv = fabsf(r.ix) == INFINITY || fabsf(r.iy) == INFINITY || fabsf(r.iz) == INFINITY;
+            bool got = __ballot(inf_inv) ? traverse<W, ANY, LST, BLOCK, COUNT, true>(S, r, st, c)
+                                          : traverse<W, ANY, LST, BLOCK, COUNT, false>(S, r, st, c);
             ++nrays;
             if (r.overflow) atomicOr(&a.ctr->q[7], 1u);
             if (ANY) {
