@@ -1322,3 +1322,19 @@ int or_ray_tri(const yn_triangle* tri, const float o[4], const float d[4], float
     *len = r.len;
     return got;
 }
+
+/* createRay of a pixel's first sample (udpt.cl:175-189, 213-238) */
+void or_primary_ray(const yn_cam* cam, int W, int H, uint32_t rand, int x, int y, float o[4], float d[4])
+{
+    uint32_t seed = or_pixel_seed(rand, x, y, W);
+    float r1 = next_rand(&seed);
+    float r2 = next_rand(&seed);
+    ray_t r = create_ray((float)x + r1, (float)y + r2, W, H, cam);
+    S4(r.o, o);
+    S4(r.d, d);
+}
+
+float or_phong_pdf(const yn_material* m, const float n[4], const float w_i[4], const float w_o[4])
+{
+    return phong_pdf(m, A4(w_i), A4(w_o), A4(n));
+}

@@ -110,6 +110,10 @@ def lib():
                                        C.POINTER(C.c_float), C.c_void_p]
         L.or_ray_box.restype = C.c_int
         L.or_ray_box.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_primary_ray.argtypes = [C.POINTER(Cam), C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_void_p,
+                                     C.c_void_p]
+        L.or_phong_pdf.restype = C.c_float
+        L.or_phong_pdf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_ray_tri.restype = C.c_int
         L.or_ray_tri.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
         _o = L
