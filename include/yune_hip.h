@@ -47,7 +47,7 @@ typedef struct yn_config {
     int32_t max_paths;        /* paths in flight per batch (0: default, 8M)         */
     size_t  mem_budget;       /* bytes for job buffers; 0: 64 GiB                   */
     int32_t bvh_width;        /* GPU node width 2 or 4 (0: 4)                       */
-    int32_t trace_block;      /* trace kernel block size 64/128/256 (0: 128)        */
+    int32_t trace_block;      /* trace kernel block size 64/128/256 (0: 256)        */
     int32_t trace_chunk;      /* rays per wave per queue fetch (0: 64)              */
     int32_t trace_grid;       /* trace blocks per launch (0: 8 per CU)              */
     float   hoist_frac;       /* triangles whose box area exceeds this fraction of the
@@ -80,7 +80,9 @@ typedef struct yn_host_scene {
     yn_light_quad light;
     yn_cam cam;
 } yn_host_scene;
-/* Scene::loadModel OBJ/MTL reader (src/Scene.cpp:133-410). */
+/* Scene::loadModel OBJ/MTL reader (src/Scene.cpp:133-410).  The camera is
+ * the reference's default (Camera.cpp:93-103) and the light the kernels'
+ * constant quad (udpt.cl:96-104), as the reference renders an OBJ scene. */
 int  yn_load_obj(const char* obj_path, yn_host_scene* out);
 /* Procedural scenes: Cornell box (36 triangles) and icosphere rooms. */
 int  yn_gen_cornell(yn_host_scene* out);

@@ -52,10 +52,16 @@ int yn_load_obj(const char* path, yn_host_scene* out)
     std::string err;
     int r = load_obj(path, h, err);
     if (r) { set_global_error(err); return r; }
-    // reference default camera (Camera.cpp:93-103) and no light: the caller
-    // supplies the kernel's light quad.
+    // reference default camera (Camera.cpp:93-103) and the kernels' constant
+    // light quad (udpt.cl:96-104, bdpt.cl same), which every OBJ scene of
+    // the reference is rendered with; callers may replace it.
     float side[3] = {1, 0, 0}, up[3] = {0, 1, 0}, look[3] = {0, 0, -1}, eye[3] = {0, 0, 0};
     make_camera(side, up, look, eye, 60.0f, h.cam);
+    h.light.pos = {-0.1979f, 0.92f, -3.1972f, 1.f};
+    h.light.normal = {0.f, -1.f, 0.f, 0.f};
+    h.light.ke = {16.f, 16.f, 16.f, 0.f};
+    h.light.edge_l = {0.4f, 0.f, 0.f, 0.f};
+    h.light.edge_w = {0.f, 0.f, 0.4f, 0.f};
     return to_c(h, out);
 }
 
