@@ -54,6 +54,37 @@ def cpu_baseline(scene, nodes, threads: int, budget_s: float = 12.0):
                       f"{samples} samples, {rays} BVH queries in {dt:.1f} s, {threads} threads"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_ext_1m_1080p_traffic.txt")
+
+
+def copy_probe(n_bytes: int = 1 << 30, reps: int = 3) -> None:
+    """A plain device-to-device copy of known size (torch's copy kernel), for
+    scaling the FETCH_SIZE / WRITE_SIZE counters: under `rocprofv3 --pmc`
+    each dispatch should read and write n_bytes."""
+    import torch
+    src = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n_bytes, "dispatches": reps}))
+
+
+def measured_traffic():
+    """HBM bytes per closest-hit trace launch from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    try:
+        vals = {}
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if ":" in line and not line.startswith(("#", " ")):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = v.strip()
+        return float(vals["traffic_bytes_per_launch"]), float(vals["traffic_bytes_per_ray"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,7 +95,11 @@ def main():
     ap.add_argument("--trace-block", type=int, default=0)
     ap.add_argument("--trace-chunk", type=int, default=0)
     ap.add_argument("--trace-grid", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU design is described but "
               "not built", file=sys.stderr)
@@ -142,12 +177,17 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
+            "traffic_per_ray": None,
             "bytes_per_ray": round(ext_bytes_per_ray, 1),
             "nodes_per_ray": round(cst.nodes_ext / max(cst.rays_ext, 1), 2),
             "tri_tests_per_ray": round(cst.tris_ext / max(cst.rays_ext, 1), 2),
             "avg_launch_ms": round(ext_ms / max(launches, 1), 4),
         },
     }
+    tr = measured_traffic()
+    if tr is not None:
+        line["roofline"]["traffic"], line["roofline"]["traffic_per_ray"] = round(tr[0]), round(tr[1], 1)
+        line["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         try:

@@ -525,3 +525,17 @@ def test_trace_matches_brute_force():
     t_c, _, tt_c, fl = capped.trace(rays)
     ok = (fl & 1) == 0
     assert np.array_equal(t_c[ok], t_e[ok])
+
+
+def test_bdpt_furnace_closed_form():
+    """bdpt.cl shading in an emissive grey box whose light emits nothing:
+    every eye vertex i adds T_i (ke + evaluateDirectLighting) w_i with
+    T_i = a^i, w_i = 0.1^i (ks floor 0.1, bdpt.cl:592) and NEE returning
+    the emission, so B eye vertices give 2e (1 - (0.1 a)^B) / (1 - 0.1 a)."""
+    a, e, B = 0.5, 0.2, 5
+    s = box_scene(a, e)
+    cam = O.camera((1, 0, 0), (0, 1, 0), (0, 0, -1), (0.1, 0.2, 0.3), 60.0)
+    p = O.params(1, bdpt_bounces=B, rr_threshold=100)
+    img, _, _, _ = s.render(cam, 16, 16, p, O.mt19937(2, 2))
+    exp = 2 * e * (1 - (0.1 * a) ** B) / (1 - 0.1 * a)
+    assert np.allclose(img[..., :3], exp, rtol=2e-5)

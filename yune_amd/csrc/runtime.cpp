@@ -93,6 +93,12 @@ struct yn_ctx {
     int ev_used = 0;
     int pt_job_slot = -1;             // job slot used by yn_pathtracer
     size_t job_bytes = 0;
+    // bidirectional mode: vertex arrays, wide shadow queues, path lists
+    BdptSoA B{};
+    RayQueue ShB[2]{};
+    uint32_t* lst[2] = {nullptr, nullptr};
+    int64_t cap_bdpt = 0;
+    std::vector<void*> ballocs;
     // traversal stack overflow area (entries beyond the LDS part)
     uint32_t* ovf_ref = nullptr;
     uint16_t* ovf_t = nullptr;
@@ -175,8 +181,62 @@ int alloc_wavefront(yn_ctx* c, int64_t n)
     return 0;
 }
 
+constexpr size_t kBdptBudget = (size_t)4 << 30;   // bytes for bidirectional path state
+constexpr int kMaxBdptBounces = 64;
+
+void free_bdpt(yn_ctx* c)
+{
+    for (void* p : c->ballocs) hipFree(p);
+    c->ballocs.clear();
+    c->B = BdptSoA{};
+    c->ShB[0] = c->ShB[1] = RayQueue{};
+    c->lst[0] = c->lst[1] = nullptr;
+    c->cap_bdpt = 0;
+}
+
+// Buffers of the bidirectional tracer for subpaths of up to `bounces`
+// vertices: K vertices per side, 2^shift >= K shadow-ray slots per path and
+// round, as many paths as fit the budget (and the unidirectional capacity,
+// whose path state and extension queues are shared).
+int ensure_bdpt(yn_ctx* c, int bounces)
+{
+    int K = std::max(bounces, 2);
+    if (c->cap_bdpt > 0 && c->B.K >= K) return 0;
+    free_bdpt(c);
+    int shift = 0;
+    while ((1 << shift) < K) ++shift;
+    const int64_t R = (int64_t)1 << shift;
+    const int64_t per_path = 2 * K * 64 + R * 4 + 2 * R * 32 + 2 * 4 + 3 * 4 + 2 * 16;
+    int64_t cap = std::min<int64_t>(c->cap_paths, (int64_t)(kBdptBudget / (size_t)per_path));
+    cap = std::min<int64_t>(cap, ((int64_t)1 << (31 - shift)) - 1);
+    if (cap < 1) return fail(c, YN_ERR_NO_MEMORY, "no room for bidirectional paths");
+    auto A = [&](auto** p, size_t n) -> int {
+        int r = dalloc(c, p, n);
+        if (r == 0) c->ballocs.push_back(*p);
+        return r;
+    };
+    BdptSoA& B = c->B;
+    const size_t n = (size_t)cap, nv = (size_t)(2 * K) * n, ns = n << shift;
+    int r = 0;
+    r |= A(&B.eye_o, n); r |= A(&B.eye_d, n); r |= A(&B.lp_len, n); r |= A(&B.ep_len, n);
+    r |= A(&B.w_eye, n); r |= A(&B.conn, n);
+    r |= A(&B.vp, nv); r |= A(&B.vn, nv); r |= A(&B.vd, nv); r |= A(&B.vc, nv);
+    r |= A(&B.occ, ns);
+    for (int k = 0; k < 2; ++k) {
+        r |= A(&c->ShB[k].o_t, ns); r |= A(&c->ShB[k].d_p, ns);
+        r |= A(&c->lst[k], n);
+    }
+    if (r) { free_bdpt(c); return r < 0 ? r : YN_ERR_HIP; }
+    B.K = K;
+    B.shift = shift;
+    B.cap = (int32_t)cap;
+    c->cap_bdpt = cap;
+    return 0;
+}
+
 void free_all(yn_ctx* c)
 {
+    free_bdpt(c);
     for (void* p : c->allocs) hipFree(p);
     c->allocs.clear();
     for (auto& s : c->scenes)
@@ -198,7 +258,8 @@ DevJob make_devjob(const JobRec& j)
 
 bool valid_params(const yn_render_params* p)
 {
-    return p && (p->mode == YN_MODE_PT || p->mode == YN_MODE_BDPT) && p->max_bounces >= 0 && p->bdpt_bounces >= 1;
+    return p && (p->mode == YN_MODE_PT || p->mode == YN_MODE_BDPT) && p->max_bounces >= 0 && p->bdpt_bounces >= 1 &&
+           p->bdpt_bounces <= kMaxBdptBounces;
 }
 
 bool valid_cam(const yn_cam* c)
@@ -225,7 +286,8 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         segs.push_back(g);
     }
     if (total == 0) return 0;
-    if (total > c->cap_paths) return fail(c, YN_ERR_INVALID, "batch larger than the path capacity");
+    const bool bdpt = c->jobs[regs[0].job].p.mode == YN_MODE_BDPT;
+    if (total > (bdpt ? c->cap_bdpt : c->cap_paths)) return fail(c, YN_ERR_INVALID, "batch larger than the path capacity");
     if ((int)segs.size() > c->seg_cap) {
         if (c->d_segs) hipFree(c->d_segs);
         c->d_segs = nullptr;
@@ -240,8 +302,14 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     const DevScene* scenes = c->d_scenes;
     uint32_t* q = c->ctr->q;
     // counters: q[0],q[1] extension queue counts, q[2],q[3] shadow queue counts,
-    // q[4],q[5] work counters of the extend / shadow launches.
-    for (int k = 0; k < 2; ++k) { c->E[k].count = q + k; c->Sh[k].count = q + 2 + k; }
+    // q[4],q[5] work counters of the extend / shadow launches, q[7] stack
+    // overflow flag, q[8],q[9] path-list counts (bidirectional mode).
+    for (int k = 0; k < 2; ++k) {
+        c->E[k].count = q + k;
+        c->Sh[k].count = q + 2 + k;
+        c->ShB[k].count = q + 2 + k;
+    }
+    RayQueue* SQ = bdpt ? c->ShB : c->Sh;
 
     GenArgs ga{};
     ga.scenes = scenes;
@@ -251,7 +319,14 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     ga.segs = c->d_segs;
     ga.n_segs = (int32_t)segs.size();
     ga.n_paths = (int32_t)total;
-    launch_generate(ga, cap, s);
+    if (bdpt) {
+        ga.B = c->B;
+        ga.list_out = c->lst[0];
+        ga.list_count = q + 8;
+        launch_generate_bdpt(ga, cap, s);
+    } else {
+        launch_generate(ga, cap, s);
+    }
 
     // single scene fast path
     int single = -1;
@@ -301,7 +376,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             ta.occ_out = nullptr;
             ta.work = q + 4;
             ta.zero0 = c->E[nxt].count;
-            ta.zero1 = c->Sh[nxt].count;
+            ta.zero1 = SQ[nxt].count;
             bool timed = (int)spans.size() + 3 < kEvPool / 2;
             hipEvent_t e0 = timed ? ev() : nullptr, e1 = timed ? ev() : nullptr;
             if (timed) hipEventRecord(e0, s);
@@ -309,38 +384,57 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
                 return fail(c, YN_ERR_INVALID, "BVH too deep for the traversal stack");
             if (timed) { hipEventRecord(e1, s); spans.push_back({e0, e1, 0}); }
             // shadow rays
-            ta.q = c->Sh[cur];
+            ta.q = SQ[cur];
             ta.hit_out = nullptr;
-            ta.occ_out = c->P.occ;
+            ta.occ_out = bdpt ? c->B.occ : c->P.occ;
+            ta.id_shift = bdpt ? c->B.shift : 0;
             ta.work = q + 5;
-            ta.zero0 = nullptr;
+            ta.zero0 = bdpt ? q + 8 + nxt : nullptr;    // the next path list
             ta.zero1 = nullptr;
             hipEvent_t e2 = timed ? ev() : nullptr, e3 = timed ? ev() : nullptr;
             if (timed) hipEventRecord(e2, s);
             launch_trace(ta, true, width, max_depth, block, grid, count, s);
+            ta.id_shift = 0;
             if (timed) { hipEventRecord(e3, s); spans.push_back({e2, e3, 1}); }
             // shade
-            ShadeArgs sa{};
-            sa.scenes = scenes;
-            sa.jobs = c->d_jobs;
-            sa.P = c->P;
-            sa.ext_in = c->E[cur];
-            sa.sh_in = c->Sh[cur];
-            sa.ext_out = c->E[nxt];
-            sa.sh_out = c->Sh[nxt];
-            sa.zero0 = q + 4;
-            sa.zero1 = q + 5;
-            sa.ctr = c->ctr;
             hipEvent_t e4 = timed ? ev() : nullptr, e5 = timed ? ev() : nullptr;
             if (timed) hipEventRecord(e4, s);
-            launch_shade_pt(sa, cap, shade_grid, s);
+            if (bdpt) {
+                BdptShadeArgs ba{};
+                ba.scenes = scenes;
+                ba.jobs = c->d_jobs;
+                ba.P = c->P;
+                ba.B = c->B;
+                ba.list_in = c->lst[cur];
+                ba.list_in_count = q + 8 + cur;
+                ba.list_out = c->lst[nxt];
+                ba.list_out_count = q + 8 + nxt;
+                ba.ext_out = c->E[nxt];
+                ba.sh_out = c->ShB[nxt];
+                ba.zero0 = q + 4;
+                ba.zero1 = q + 5;
+                launch_shade_bdpt(ba, cap, shade_grid, s);
+            } else {
+                ShadeArgs sa{};
+                sa.scenes = scenes;
+                sa.jobs = c->d_jobs;
+                sa.P = c->P;
+                sa.ext_in = c->E[cur];
+                sa.sh_in = c->Sh[cur];
+                sa.ext_out = c->E[nxt];
+                sa.sh_out = c->Sh[nxt];
+                sa.zero0 = q + 4;
+                sa.zero1 = q + 5;
+                sa.ctr = c->ctr;
+                launch_shade_pt(sa, cap, shade_grid, s);
+            }
             if (timed) { hipEventRecord(e5, s); spans.push_back({e4, e5, 2}); }
         }
         HIPCK(c, hipGetLastError());
         HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, s));
         HIPCK(c, hipStreamSynchronize(s));
         int nxt = it & 1;
-        if (c->h_ctr->q[nxt] == 0 && c->h_ctr->q[2 + nxt] == 0) break;
+        if (bdpt ? c->h_ctr->q[8 + nxt] == 0 : (c->h_ctr->q[nxt] == 0 && c->h_ctr->q[2 + nxt] == 0)) break;
         if (c->h_ctr->q[7]) return fail(c, YN_ERR_INVALID, "traversal stack overflow");
         if (it > 200000) return fail(c, YN_ERR_HIP, "wavefront loop did not drain");
     }
@@ -383,9 +477,14 @@ int render_rect(yn_ctx* c, int jid, float4* accum, int x0, int y0, int w, int h,
     int64_t npix = (int64_t)w * h;
     if (npix == 0 || seeds.empty()) return 0;
     int64_t full = (int64_t)J.W * J.H;
+    int64_t pcap = c->cap_paths;
+    if (J.p.mode == YN_MODE_BDPT) {
+        if (int r = ensure_bdpt(c, J.p.bdpt_bounces)) return r;
+        pcap = c->cap_bdpt;
+    }
     // Sample slots are indexed [slot][W*H]; a batch may hold K frames when
     // K full images fit in the path capacity.
-    int64_t kcap = c->cap_paths / std::max<int64_t>(full, 1);
+    int64_t kcap = pcap / std::max<int64_t>(full, 1);
     size_t f = 0;
     while (f < seeds.size()) {
         std::vector<Region> regs;
@@ -402,8 +501,8 @@ int render_rect(yn_ctx* c, int jid, float4* accum, int x0, int y0, int w, int h,
             launch_resolve(ra, c->stream);
         } else {
             // image larger than the path capacity: row bands, one frame at a time
-            if ((int64_t)J.W > c->cap_paths) return fail(c, YN_ERR_INVALID, "image row wider than the path capacity");
-            int rows = (int)std::max<int64_t>(1, c->cap_paths / J.W);
+            if ((int64_t)J.W > pcap) return fail(c, YN_ERR_INVALID, "image row wider than the path capacity");
+            int rows = (int)std::max<int64_t>(1, pcap / J.W);
             nslots = 1;
             for (int yb = y0; yb < y0 + h; yb += rows) {
                 int hh = std::min(rows, y0 + h - yb);
@@ -631,7 +730,6 @@ int yn_job_create(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t 
     if (check_scene(c, sid)) return YN_ERR_NOT_FOUND;
     if (!valid_cam(cam)) return fail(c, YN_ERR_INVALID, "camera row 4 must be (0,0,0,1)");
     if (!valid_params(p)) return fail(c, YN_ERR_INVALID, "bad render parameters");
-    if (p->mode != YN_MODE_PT) return fail(c, YN_ERR_INVALID, "BDPT mode is not available in this build");
     if (W <= 0 || H <= 0 || (int64_t)W * H > (1ll << 28) || !job_id) return fail(c, YN_ERR_INVALID, "bad image size");
     size_t bytes = (size_t)W * H * sizeof(float4);
     if (c->job_bytes + bytes > c->cfg.mem_budget) return fail(c, YN_ERR_NO_MEMORY, "job memory budget exceeded");
@@ -810,7 +908,7 @@ int yn_pathtracer(yn_ctx* c, float* out_img, const float* in_img, const yn_cam* 
     if (check_scene(c, sid)) return YN_ERR_NOT_FOUND;
     if (!out_img || (!in_img && !reset)) return fail(c, YN_ERR_INVALID, "null image");
     if (!valid_cam(cam)) return fail(c, YN_ERR_INVALID, "camera row 4 must be (0,0,0,1)");
-    if (!valid_params(params) || params->mode != YN_MODE_PT) return fail(c, YN_ERR_INVALID, "bad render parameters");
+    if (!valid_params(params)) return fail(c, YN_ERR_INVALID, "bad render parameters");
     if (W <= 0 || H <= 0 || block_x <= 0 || block_y <= 0 || block < 0 || block >= block_x * block_y)
         return fail(c, YN_ERR_INVALID, "bad image or block arguments");
     // tile of `block` (udpt.cl:166-170)

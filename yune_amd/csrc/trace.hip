@@ -304,7 +304,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a)
             float4 ot = a.q.o_t[i];
             float4 dp = a.q.d_p[i];
             int pidx = __float_as_int(dp.w);
-            int sc = a.single_scene >= 0 ? a.single_scene : a.jobs[a.path_job[pidx]].scene;
+            int sc = a.single_scene >= 0 ? a.single_scene : a.jobs[a.path_job[pidx >> a.id_shift]].scene;
             const DevScene& S = a.scenes[sc];
             RayState r;
             r.ox = ot.x; r.oy = ot.y; r.oz = ot.z;

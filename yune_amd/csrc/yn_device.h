@@ -72,9 +72,29 @@ struct PathSoA {
     int32_t* occ;               // any-hit result
 };
 
+// Bidirectional paths (bdpt.cl): both subpaths' vertices plus the
+// per-path bookkeeping of the connection rounds.  Vertex arrays are
+// [side * K + k][cap] (side 0: light subpath, 1: eye subpath).
+struct BdptSoA {
+    float4* eye_o;              // camera ray, kept until the eye subpath starts
+    float4* eye_d;
+    int32_t* lp_len;            // light subpath length (vertex 0 lies on the quad)
+    int32_t* ep_len;            // eye subpath length
+    float* w_eye;               // eye_path_weight (bdpt.cl:568)
+    uint32_t* conn;             // shadow-ray slots issued in the current round
+    float4* vp;                 // vertex position, w = triangle id bits (-1: on the light)
+    float4* vn;                 // shading normal
+    float4* vd;                 // direction of the ray that found the vertex
+    float4* vc;                 // contribution (PathInfo.contrib)
+    int32_t* occ;               // [cap << shift] any-hit results of a round
+    int32_t K;                  // vertices kept per subpath
+    int32_t shift;              // log2 of shadow-ray slots per path and round
+    int32_t cap;                // paths
+};
+
 // Counters and stats in device memory.
 struct DevCounters {
-    uint32_t q[8];              // queue counts, work counters (see runtime)
+    uint32_t q[12];             // queue counts, work counters (see runtime)
     unsigned long long rays_ext;
     unsigned long long rays_shadow;
     unsigned long long nodes_ext;     // counting mode only: interior nodes visited

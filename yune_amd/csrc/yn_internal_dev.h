@@ -24,6 +24,7 @@ struct TraceArgs {
     uint32_t* ovf_ref;          // traversal stack overflow area: ovf_depth entries
     uint16_t* ovf_t;            //   per grid thread, strided by the grid size
     int32_t ovf_depth;
+    int32_t id_shift;           // ray id >> id_shift = path index (result slot = ray id)
 };
 
 // Segment of primary rays: a rectangle of one job's image at one frame.
@@ -45,6 +46,23 @@ struct ShadeArgs {
     DevCounters* ctr;
 };
 
+// Shade pass of the bidirectional tracer: advances the paths of `list_in`
+// (one entry per path whose rays came back) and lists those that issued
+// new rays in `list_out`.
+struct BdptShadeArgs {
+    const DevScene* scenes;
+    const DevJob* jobs;
+    PathSoA P;
+    BdptSoA B;
+    const uint32_t* list_in;
+    const uint32_t* list_in_count;
+    uint32_t* list_out;
+    uint32_t* list_out_count;
+    RayQueue ext_out, sh_out;
+    uint32_t* zero0;            // work counters to clear for the next traces
+    uint32_t* zero1;
+};
+
 struct GenArgs {
     const DevScene* scenes;
     const DevJob* jobs;
@@ -53,6 +71,9 @@ struct GenArgs {
     const GenSeg* segs;
     int32_t n_segs;
     int32_t n_paths;
+    BdptSoA B;                  // bidirectional mode only
+    uint32_t* list_out;         // bidirectional mode: every path starts listed
+    uint32_t* list_count;
 };
 
 struct ResolveArgs {
@@ -81,6 +102,24 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
+// Wave-aggregated reservation of n_i consecutive slots per lane (lanes in
+// order), one atomic per wave.  Every lane of the wave must call it.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, uint32_t n)
+{
+    const int lane = (int)__lane_id();
+    uint32_t x = n;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    uint32_t total = __shfl(x, 63);
+    if (total == 0) return 0;
+    uint32_t base = 0;
+    if (lane == 63) base = atomicAdd(counter, total);
+    base = __shfl(base, 63);
+    return base + x - n;
+}
+
 // Wave-aggregated append: one atomic per wave, lanes get consecutive slots
 // in lane order.  Every active lane of the wave must call it.
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
@@ -102,6 +141,8 @@ int launch_trace(const TraceArgs& a, bool any_hit, int width, int stack, int blo
 int trace_lds_stack();          // traversal stack entries kept in LDS
 void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t stream);
 void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, hipStream_t stream);
+void launch_generate_bdpt(const GenArgs& a, const DevCapture& cap, hipStream_t stream);
+void launch_shade_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int grid, hipStream_t stream);
 void launch_resolve(const ResolveArgs& a, hipStream_t stream);
 void launch_tonemap(const float4* in, float4* out, int n, hipStream_t stream);
 void launch_prepare(const DevScene* scene, const float* rays, int n, RayQueue q, int32_t* light_out,
