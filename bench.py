@@ -1,12 +1,21 @@
-"""Benchmark: Mrays/s of the path-tracing hot path on the ~1M-triangle
-1920x1080 scene (48 icospheres of 20,480 triangles in a 10x10x10 room,
-2-triangle quad light; max depth 8, Russian roulette from bounce 3, NEE+MIS).
+"""Benchmark: Mrays/s of the path-tracing hot path on one MI355X.
 
+Default workload (the headline): the ~1M-triangle 1920x1080 scene (48
+icospheres of 20,480 triangles in a 10x10x10 room, 2-triangle quad light),
+path tracing with NEE+MIS, max depth 8, Russian roulette from bounce 3.
 A step is one batch of the wavefront path tracer: 4 frames of 1920x1080 at
 1 sample per pixel each (8.3M paths) run from primary rays until every path
-has ended, then folded into the progressive image.  16 steps are the 64 spp
-of the workload.  Rays counted: every BVH query the GPU performs (extension,
-NEE shadow, MIS any-hit); the CPU baseline counts with the same rule.
+has ended, then folded into the progressive image; 16 steps are the 64 spp
+of the workload.
+
+Other single-GPU workloads (--workload):
+  cornell  the 36-triangle Cornell box, 512x512, NEE+MIS, depth 8, RR from
+           bounce 3; 32 frames per step, 32 steps = 1024 spp
+  bdpt     bidirectional (bdpt.cl semantics) on the Cornell box, 1280x720,
+           5-vertex subpaths, MIS; 4 frames per step, 8 steps = 32 spp
+
+Rays counted: every BVH query the GPU performs (extension, NEE shadow, MIS
+any-hit, BDPT connection); the CPU baseline counts with the same rule.
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
@@ -21,40 +30,57 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-W, H = 1920, 1080
-SCENE_SEED = 2024
 BASE_SEED = 1
-FRAMES_PER_STEP = 4
-PARAMS = dict(mis=1, max_bounces=8, rr_threshold=2)
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_ext_1m_1080p_traffic.txt")
+
+WORKLOADS = {
+    "icosphere": dict(
+        metric="Mrays/s at 1M tris 1080p", W=1920, H=1080, frames=4, steps=16, mode=0,
+        params=dict(mis=1, max_bounces=8, rr_threshold=2),
+        desc="path tracing, 48 icospheres x 20480 tris + 12-tri room + 2-tri quad light ({n} tris), 1920x1080, "
+             "{f} spp per step, max depth 8, RR from bounce 3, NEE+MIS"),
+    "cornell": dict(
+        metric="Mrays/s Cornell box 512x512", W=512, H=512, frames=32, steps=32, mode=0,
+        params=dict(mis=1, max_bounces=8, rr_threshold=2),
+        desc="path tracing, Cornell box ({n} tris), 512x512, {f} spp per step, max depth 8, RR from bounce 3, "
+             "NEE+MIS"),
+    "bdpt": dict(
+        metric="Mrays/s BDPT Cornell box 720p", W=1280, H=720, frames=4, steps=8, mode=1,
+        params=dict(mis=1, bdpt_bounces=5),
+        desc="bidirectional path tracing (bdpt.cl), Cornell box ({n} tris), 1280x720, {f} spp per step, "
+             "5-vertex eye and light subpaths, all connections, NEE+MIS"),
+}
 
 
-def cpu_baseline(scene, nodes, threads: int, budget_s: float = 12.0):
-    """The CPU restatement of udpt.cl (reference BFS traversal, HEAP_SIZE
-    1500) on the same scene, camera and seeds, over a centre tile."""
-    import numpy as np
+def make_scene(Y, name):
+    if name == "icosphere":
+        return Y.gen_icosphere_room(2024, 48, 5)
+    return Y.gen_cornell()
 
+
+def cpu_baseline(scene, nodes, wl, threads: int, budget_s: float = 12.0):
+    """The CPU restatement of udpt.cl / bdpt.cl (reference BFS traversal,
+    HEAP_SIZE 1500) on the same scene, camera and seeds, over a centre tile."""
     from oracle import pyoracle as O
+    W, H = wl["W"], wl["H"]
     osc = O.OScene(scene.tris, scene.mats, nodes, scene.light, heap_size=1500)
     cam = O.Cam.from_buffer_copy(bytes(scene.cam))
-    p = O.params(0, **PARAMS)
+    p = O.params(wl["mode"], **wl["params"])
     tw, th = 128, 128
     x0, y0 = W // 2 - tw // 2, H // 2 - th // 2
     rect = (x0, y0, x0 + tw, y0 + th)
     seeds = O.mt19937(BASE_SEED, 64)
     t0 = time.perf_counter()
-    _, _, rays1, _ = osc.render(cam, W, H, p, seeds[:1], rect=rect, threads=threads)
+    osc.render(cam, W, H, p, seeds[:1], rect=rect, threads=threads)
     t1 = time.perf_counter() - t0
     spp = int(max(1, min(63, round(budget_s / max(t1, 1e-3)))))
     t0 = time.perf_counter()
     _, _, rays, samples = osc.render(cam, W, H, p, seeds[1:1 + spp], rect=rect, threads=threads)
     dt = time.perf_counter() - t0
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{tw}x{th} centre tile of the same 1920x1080 frame, {spp} spp (frames 2..{1 + spp}), "
+    return {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{tw}x{th} centre tile of the same {W}x{H} frame, {spp} spp (frames 2..{1 + spp}), "
                       f"{samples} samples, {rays} BVH queries in {dt:.1f} s, {threads} threads"}
-
-
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_ext_1m_1080p_traffic.txt")
 
 
 def copy_probe(n_bytes: int = 1 << 30, reps: int = 3) -> None:
@@ -88,13 +114,16 @@ def measured_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the workload's default)")
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="icosphere")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--trace-block", type=int, default=0)
     ap.add_argument("--trace-chunk", type=int, default=0)
     ap.add_argument("--trace-grid", type=int, default=0)
+    ap.add_argument("--lds-stack", type=int, default=0)
+    ap.add_argument("--shade-waves", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC counters")
     args = ap.parse_args()
     if args.copy_probe:
@@ -109,31 +138,34 @@ def main():
 
     import yune_amd as Y
 
+    wl = WORKLOADS[args.workload]
+    W, H, F = wl["W"], wl["H"], wl["frames"]
+    steps = args.steps or wl["steps"]
     t0 = time.perf_counter()
-    scene = Y.gen_icosphere_room(SCENE_SEED, 48, 5)
+    scene = make_scene(Y, args.workload)
     nodes = scene.bvh()
     t_build = time.perf_counter() - t0
     print(f"# scene {len(scene.tris)} triangles, {len(nodes)} BVH nodes, build {t_build:.2f}s", file=sys.stderr,
           flush=True)
 
-    params = Y.params_reference(**PARAMS)
+    params = Y.params_reference(wl["mode"], **wl["params"])
     with Y.Context(bvh_width=args.bvh_width, trace_block=args.trace_block, trace_chunk=args.trace_chunk,
-                   trace_grid=args.trace_grid) as ctx:
+                   trace_grid=args.trace_grid, lds_stack=args.lds_stack, shade_waves=args.shade_waves) as ctx:
         sid = ctx.scene(scene, nodes)
         info = ctx.scene_info(sid)
         # counting pass: exact node visits / triangle tests of one step
         cj = ctx.job(sid, scene.cam, W, H, params, base_seed=BASE_SEED)
         cj.set_counting(True)
-        cj.render(FRAMES_PER_STEP)
+        cj.render(F)
         cst = cj.stats()
         cj.release()
         job = ctx.job(sid, scene.cam, W, H, params, base_seed=BASE_SEED)
         for _ in range(args.warmup):
-            job.render(FRAMES_PER_STEP)
+            job.render(F)
         s0 = job.stats()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            job.render(FRAMES_PER_STEP)
+        for _ in range(steps):
+            job.render(F)
         dt = time.perf_counter() - t0
         s1 = job.stats()
         img = job.read()
@@ -141,7 +173,8 @@ def main():
     rays = (s1.rays_ext - s0.rays_ext) + (s1.rays_shadow - s0.rays_shadow)
     mrays = rays / dt / 1e6
     # roofline of the dominant kernel (closest-hit traversal): algorithmic
-    # bytes per extension ray from the counting pass.
+    # bytes per extension ray from the counting pass: interior nodes visited
+    # x node size + triangle tests x 48 B + the ray (32 B) and its hit (16 B)
     node_b = 64 if info.width == 2 else 128
     ext_bytes_per_ray = (cst.nodes_ext * node_b + cst.tris_ext * 48) / max(cst.rays_ext, 1) + 32 + 16
     ext_rays = s1.rays_ext - s0.rays_ext
@@ -149,25 +182,25 @@ def main():
     launches = s1.trace_launches_ext - s0.trace_launches_ext
     achieved = ext_bytes_per_ray * ext_rays / (ext_ms / 1e3) / 1e9 if ext_ms > 0 else 0.0
     line = {
-        "metric": "Mrays/s at 1M tris 1080p",
+        "metric": wl["metric"],
         "value": round(mrays, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt * 1e3 / max(args.steps, 1), 3),
+        "ms_per_step": round(dt * 1e3 / max(steps, 1), 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "path tracing, 48 icospheres x 20480 tris + 12-tri room + 2-tri quad light "
-                        f"({len(scene.tris)} tris), 1920x1080, {FRAMES_PER_STEP} spp per step, max depth 8, "
-                        "RR from bounce 3, NEE+MIS",
+            "workload": wl["desc"].format(n=len(scene.tris), f=F),
             "triangles": len(scene.tris), "bvh_nodes_ref": len(nodes), "bvh_width": info.width,
-            "resolution": [W, H], "spp_per_step": FRAMES_PER_STEP, "spp_total": FRAMES_PER_STEP * args.steps,
+            "resolution": [W, H], "spp_per_step": F, "spp_total": F * steps,
             "rays": int(rays), "rays_per_sample": round(rays / max(s1.samples - s0.samples, 1), 3),
+            "ms_trace_ext": round(ext_ms, 2), "ms_trace_shadow": round(s1.ms_trace_shadow - s0.ms_trace_shadow, 2),
+            "ms_shade": round(s1.ms_shade - s0.ms_shade, 2),
         },
         "roofline": {
             "bound": "hbm",
@@ -181,17 +214,18 @@ def main():
             "bytes_per_ray": round(ext_bytes_per_ray, 1),
             "nodes_per_ray": round(cst.nodes_ext / max(cst.rays_ext, 1), 2),
             "tri_tests_per_ray": round(cst.tris_ext / max(cst.rays_ext, 1), 2),
+            "rays_per_launch": round(ext_rays / max(launches, 1)),
             "avg_launch_ms": round(ext_ms / max(launches, 1), 4),
         },
     }
-    tr = measured_traffic()
+    tr = measured_traffic() if args.workload == "icosphere" else None
     if tr is not None:
         line["roofline"]["traffic"], line["roofline"]["traffic_per_ray"] = round(tr[0]), round(tr[1], 1)
         line["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         try:
-            line["cpu_baseline"] = cpu_baseline(scene, nodes, threads)
+            line["cpu_baseline"] = cpu_baseline(scene, nodes, wl, threads)
         except Exception as e:  # the baseline is reported, never required
             line["cpu_baseline"] = {"value": None, "error": repr(e)}
     finite = np.isfinite(img[..., :3]).all(-1).mean()

@@ -48,7 +48,8 @@ class RenderParams(C.Structure):
 class Config(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_paths", C.c_int32), ("mem_budget", C.c_size_t),
                 ("bvh_width", C.c_int32), ("trace_block", C.c_int32), ("trace_chunk", C.c_int32),
-                ("trace_grid", C.c_int32), ("hoist_frac", C.c_float), ("leaf_max", C.c_int32)]
+                ("trace_grid", C.c_int32), ("hoist_frac", C.c_float), ("leaf_max", C.c_int32),
+                ("lds_stack", C.c_int32), ("shade_waves", C.c_int32)]
 
 
 class Stats(C.Structure):

@@ -92,7 +92,19 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
 
     int tri = P.tri[p];
     f4 hp = ld(P.hp[p]), hn = ld(P.hn[p]), rd = ld(P.rd[p]);
-    f4 tp, direct, indirect;
+    // throughput and radiance sums are read only by the steps that use them
+    // and written back only when changed (the shade pass is bound by this
+    // path-state traffic)
+    f4 tp = mk(0, 0, 0, 0), direct = mk(0, 0, 0, 0), indirect = mk(0, 0, 0, 0);
+    bool acc_loaded = false, tp_dirty = false, dir_dirty = false, ind_dirty = false, v_dirty = false;
+    auto load_acc = [&]() {
+        if (!acc_loaded) {
+            tp = ld(P.tp[p]);
+            direct = ld(P.direct[p]);
+            indirect = ld(P.indirect[p]);
+            acc_loaded = true;
+        }
+    };
     int bounce = P.bounce[p];
 
     // NEE scratch
@@ -132,15 +144,13 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
             tp = mk(1, 1, 1, 1);
             direct = mk(0, 0, 0, 0);
             indirect = mk(0, 0, 0, 0);
+            acc_loaded = tp_dirty = dir_dirty = ind_dirty = v_dirty = true;
             bounce = 0;
             Mat m = load_mat(S.mats, S.tri_mat[tri]);
             if (!m.spec) { st |= F_DIRECT; step = S_NEE_START; }
             else { step = S_BOUNCE_SAMPLE; }
         }
     } else {
-        tp = ld(P.tp[p]);
-        direct = ld(P.direct[p]);
-        indirect = ld(P.indirect[p]);
         if (code == ST_NEE_SH) {
             occ = P.occ[p];
             step = S_NEE_SHADOW;
@@ -153,8 +163,9 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
             Mat m = load_mat(S.mats, S.tri_mat[tri]);
             f4 nd = ld(P.nd[p]);
             float4 bnc = P.bnc[p];
+            load_acc();
             if (htri < 0) {
-                if ((st & F_LCAND) && m.spec) indirect = indirect + tp * ld(L.ke);
+                if ((st & F_LCAND) && m.spec) { indirect = indirect + tp * ld(L.ke); ind_dirty = true; }
                 step = S_FINISH;
             } else {
                 f4 o = mk(ev_o.x, ev_o.y, ev_o.z, 1.0f);
@@ -162,6 +173,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
                 f4 nhn = shade_normal(S.tri_shade, htri, h.z, h.w);
                 Mat nm = load_mat(S.mats, S.tri_mat[htri]);
                 indirect = indirect + tp * nm.ke;
+                ind_dirty = tp_dirty = true;
                 if (m.spec) tp = tp * bnc.z;
                 else
                     tp = ((tp * eval_brdf(m, nd, -rd, hp, hn, (st & F_BGLOSSY) != 0, bnc.y, on, false)) *
@@ -174,6 +186,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
                     else tp = tp * (1 / pr);
                 }
                 if (step != S_FINISH) {
+                    v_dirty = true;
                     tri = htri;
                     hp = nhp;
                     hn = nhn;
@@ -276,8 +289,9 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
             continue;
         }
         if (step == S_NEE_DONE) {
-            if (st & F_DIRECT) direct = nee_res;
-            else indirect = indirect + tp * nee_res;
+            load_acc();
+            if (st & F_DIRECT) { direct = nee_res; dir_dirty = true; }
+            else { indirect = indirect + tp * nee_res; ind_dirty = true; }
             step = S_BOUNCE_SAMPLE;
             continue;
         }
@@ -305,7 +319,13 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
             out = make_ray(1, no, nd, tmax, p);
             break;
         }
-        if (step == S_FINISH) { write_sample(J, slot, pix, direct + indirect); st = ST_DONE; ended = true; break; }
+        if (step == S_FINISH) {
+            load_acc();
+            write_sample(J, slot, pix, direct + indirect);
+            st = ST_DONE;
+            ended = true;
+            break;
+        }
         if (step == S_FINISH_CONST) { write_sample(J, slot, pix, const_color); st = ST_DONE; ended = true; break; }
     }
     if (out.kind == 0 && !ended) {   // unreachable by construction; never leave a path hanging
@@ -316,21 +336,26 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
     P.state[p] = st;
     P.seed[p] = seed;
     if (out.kind != 0) {
-        P.tri[p] = tri;
-        P.bounce[p] = bounce;
-        P.hp[p] = st4(hp);
-        P.hn[p] = st4(hn);
-        P.rd[p] = st4(rd);
-        P.tp[p] = st4(tp);
-        P.direct[p] = st4(direct);
-        P.indirect[p] = st4(indirect);
+        if (v_dirty) {
+            P.tri[p] = tri;
+            P.bounce[p] = bounce;
+            P.hp[p] = st4(hp);
+            P.hn[p] = st4(hn);
+            P.rd[p] = st4(rd);
+        }
+        if (tp_dirty) P.tp[p] = st4(tp);
+        if (dir_dirty) P.direct[p] = st4(direct);
+        if (ind_dirty) P.indirect[p] = st4(indirect);
     }
     return out;
 }
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_shade_pt(ShadeArgs a, DevCapture cap)
+// WAVES: the occupancy the register allocation is held to (2: no limit on
+// this kernel's ~230 VGPRs; 3 and 4 trade spills for latency hiding).
+template <int WAVES>
+__global__ __launch_bounds__(256, WAVES) void k_shade_pt(ShadeArgs a, DevCapture cap)
 {
     const uint32_t n_ext = *a.ext_in.count, n_sh = *a.sh_in.count;
     const uint32_t total = n_ext + n_sh;
@@ -458,9 +483,11 @@ void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t s)
     int grid = (a.n_paths + 255) / 256;
     if (grid > 0) hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, a, cap);
 }
-void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, hipStream_t s)
+void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, int waves, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shade_pt, dim3(grid), dim3(256), 0, s, a, cap);
+    if (waves == 4) hipLaunchKernelGGL(k_shade_pt<4>, dim3(grid), dim3(256), 0, s, a, cap);
+    else if (waves == 3) hipLaunchKernelGGL(k_shade_pt<3>, dim3(grid), dim3(256), 0, s, a, cap);
+    else hipLaunchKernelGGL(k_shade_pt<2>, dim3(grid), dim3(256), 0, s, a, cap);
 }
 void launch_resolve(const ResolveArgs& a, hipStream_t s)
 {

@@ -345,7 +345,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     int block = c->cfg.trace_block;
     int grid = c->cfg.trace_grid > 0 ? c->cfg.trace_grid : c->n_cu * (2048 / block);
     int shade_grid = c->n_cu * 8;
-    if (int r = ensure_overflow(c, max_depth - trace_lds_stack(), (int64_t)grid * block)) return r;
+    if (int r = ensure_overflow(c, max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * block)) return r;
     bool count = stat_job.counting != 0;
 
     int ev_base = 0;
@@ -380,7 +380,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             bool timed = (int)spans.size() + 3 < kEvPool / 2;
             hipEvent_t e0 = timed ? ev() : nullptr, e1 = timed ? ev() : nullptr;
             if (timed) hipEventRecord(e0, s);
-            if (launch_trace(ta, false, width, max_depth, block, grid, count, s) != 0)
+            if (launch_trace(ta, false, width, max_depth, block, grid, count, c->cfg.lds_stack, s) != 0)
                 return fail(c, YN_ERR_INVALID, "BVH too deep for the traversal stack");
             if (timed) { hipEventRecord(e1, s); spans.push_back({e0, e1, 0}); }
             // shadow rays
@@ -393,7 +393,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             ta.zero1 = nullptr;
             hipEvent_t e2 = timed ? ev() : nullptr, e3 = timed ? ev() : nullptr;
             if (timed) hipEventRecord(e2, s);
-            launch_trace(ta, true, width, max_depth, block, grid, count, s);
+            launch_trace(ta, true, width, max_depth, block, grid, count, c->cfg.lds_stack, s);
             ta.id_shift = 0;
             if (timed) { hipEventRecord(e3, s); spans.push_back({e2, e3, 1}); }
             // shade
@@ -426,7 +426,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
                 sa.zero0 = q + 4;
                 sa.zero1 = q + 5;
                 sa.ctr = c->ctr;
-                launch_shade_pt(sa, cap, shade_grid, s);
+                launch_shade_pt(sa, cap, shade_grid, c->cfg.shade_waves, s);
             }
             if (timed) { hipEventRecord(e5, s); spans.push_back({e4, e5, 2}); }
         }
@@ -576,6 +576,11 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.hoist_frac == 0.0f) k.hoist_frac = 0.01f;
     if (k.leaf_max <= 0) k.leaf_max = 4;
     if (k.leaf_max > 10) k.leaf_max = 10;
+    if (k.lds_stack == 0) k.lds_stack = 16;
+    if (k.shade_waves == 0) k.shade_waves = 2;
+    if (k.lds_stack != 8 && k.lds_stack != 16 && k.lds_stack != 32)
+        return fail(nullptr, YN_ERR_INVALID, "lds_stack must be 8, 16 or 32");
+    if (k.shade_waves < 2 || k.shade_waves > 4) return fail(nullptr, YN_ERR_INVALID, "shade_waves must be 2, 3 or 4");
     if (k.bvh_width != 2 && k.bvh_width != 4) return fail(nullptr, YN_ERR_INVALID, "bvh_width must be 2 or 4");
     if (k.trace_block != 64 && k.trace_block != 128 && k.trace_block != 256)
         return fail(nullptr, YN_ERR_INVALID, "trace_block must be 64, 128 or 256");
@@ -982,12 +987,12 @@ int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri,
         ta.occ_out = any ? c->P.occ : nullptr;
         ta.work = c->ctr->q + 4;
         int grid = c->n_cu * (2048 / c->cfg.trace_block);
-        if (int rr = ensure_overflow(c, S.info.max_depth - trace_lds_stack(), (int64_t)grid * c->cfg.trace_block))
+        if (int rr = ensure_overflow(c, S.info.max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * c->cfg.trace_block))
             return rr;
         ta.ovf_ref = c->ovf_ref;
         ta.ovf_t = c->ovf_t;
         ta.ovf_depth = c->ovf_depth;
-        if (launch_trace(ta, any != 0, S.info.width, S.info.max_depth, c->cfg.trace_block, grid, false, s) != 0)
+        if (launch_trace(ta, any != 0, S.info.width, S.info.max_depth, c->cfg.trace_block, grid, false, c->cfg.lds_stack, s) != 0)
             return fail(c, YN_ERR_INVALID, "BVH too deep");
         HIPCK(c, hipGetLastError());
         HIPCK(c, hipMemcpyAsync(hits.data(), c->P.hit, (size_t)m * 16, hipMemcpyDeviceToHost, s));

@@ -293,10 +293,20 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a)
     const uint32_t n = *a.q.count;
     Counts c{0, 0};
     unsigned nrays = 0;
-    for (;;) {
+    // chunk `wave` first without an atomic, then chunks past the static ones
+    // from the shared counter: a wave with nothing to do leaves without
+    // touching it (same-address atomics from every wave of the persistent
+    // grid would otherwise set a floor on the launch time of short queues).
+    const uint32_t wave = gtid >> 6;
+    const uint32_t n_static = (gridDim.x * BLOCK) >> 6;
+    for (bool first = true;; first = false) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(a.work, (uint32_t)a.chunk);
-        base = __shfl(base, 0);
+        if (first) {
+            base = wave * (uint32_t)a.chunk;
+        } else {
+            if (lane == 0) base = atomicAdd(a.work, (uint32_t)a.chunk);
+            base = __shfl(base, 0) + n_static * (uint32_t)a.chunk;
+        }
         if (base >= n) break;
         for (uint32_t off = (uint32_t)lane; off < (uint32_t)a.chunk; off += 64) {
             uint32_t i = base + off;
@@ -342,40 +352,48 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a)
 }
 
 namespace {
-constexpr int kLdsStack = 16;
-
-template <int W, bool ANY, int BL>
+template <int W, bool ANY, int LST, int BL>
 void launch_b(const TraceArgs& a, int grid, bool count, hipStream_t s)
 {
-    if (count) hipLaunchKernelGGL((k_trace<W, ANY, kLdsStack, BL, true>), dim3(grid), dim3(BL), 0, s, a);
-    else hipLaunchKernelGGL((k_trace<W, ANY, kLdsStack, BL, false>), dim3(grid), dim3(BL), 0, s, a);
+    if (count) hipLaunchKernelGGL((k_trace<W, ANY, LST, BL, true>), dim3(grid), dim3(BL), 0, s, a);
+    else hipLaunchKernelGGL((k_trace<W, ANY, LST, BL, false>), dim3(grid), dim3(BL), 0, s, a);
 }
 
-template <bool ANY>
+template <bool ANY, int LST>
 void launch_w(const TraceArgs& a, int width, int block, int grid, bool count, hipStream_t s)
 {
     if (width == 2) {
-        if (block == 64) launch_b<2, ANY, 64>(a, grid, count, s);
-        else if (block == 128) launch_b<2, ANY, 128>(a, grid, count, s);
-        else launch_b<2, ANY, 256>(a, grid, count, s);
+        if (block == 64) launch_b<2, ANY, LST, 64>(a, grid, count, s);
+        else if (block == 128) launch_b<2, ANY, LST, 128>(a, grid, count, s);
+        else launch_b<2, ANY, LST, 256>(a, grid, count, s);
     } else {
-        if (block == 64) launch_b<4, ANY, 64>(a, grid, count, s);
-        else if (block == 128) launch_b<4, ANY, 128>(a, grid, count, s);
-        else launch_b<4, ANY, 256>(a, grid, count, s);
+        if (block == 64) launch_b<4, ANY, LST, 64>(a, grid, count, s);
+        else if (block == 128) launch_b<4, ANY, LST, 128>(a, grid, count, s);
+        else launch_b<4, ANY, LST, 256>(a, grid, count, s);
     }
+}
+
+template <bool ANY>
+void launch_l(const TraceArgs& a, int width, int block, int grid, bool count, int lds, hipStream_t s)
+{
+    if (lds == 8) launch_w<ANY, 8>(a, width, block, grid, count, s);
+    else if (lds == 32) launch_w<ANY, 32>(a, width, block, grid, count, s);
+    else launch_w<ANY, 16>(a, width, block, grid, count, s);
 }
 }  // namespace
 
-int trace_lds_stack() { return kLdsStack; }
+// LDS part of the traversal stack: 8, 16 or 32 entries per lane.
+int trace_lds_stack(int lds) { return lds == 8 || lds == 32 ? lds : 16; }
 
 // `stack`: traversal stack entries the scene can need; the launch is refused
 // when LDS plus the overflow area cannot hold them.
-int launch_trace(const TraceArgs& a, bool any_hit, int width, int stack, int block, int grid, bool count,
+int launch_trace(const TraceArgs& a, bool any_hit, int width, int stack, int block, int grid, bool count, int lds,
                  hipStream_t stream)
 {
-    if (stack > kLdsStack + a.ovf_depth) return -1;
-    if (any_hit) launch_w<true>(a, width, block, grid, count, stream);
-    else launch_w<false>(a, width, block, grid, count, stream);
+    lds = trace_lds_stack(lds);
+    if (stack > lds + a.ovf_depth) return -1;
+    if (any_hit) launch_l<true>(a, width, block, grid, count, lds, stream);
+    else launch_l<false>(a, width, block, grid, count, lds, stream);
     return 0;
 }
 

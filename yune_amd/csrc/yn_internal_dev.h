@@ -136,11 +136,11 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
 }
 #endif
 
-int launch_trace(const TraceArgs& a, bool any_hit, int width, int stack, int block, int grid, bool count,
+int launch_trace(const TraceArgs& a, bool any_hit, int width, int stack, int block, int grid, bool count, int lds,
                  hipStream_t stream);
-int trace_lds_stack();          // traversal stack entries kept in LDS
+int trace_lds_stack(int lds);   // traversal stack entries kept in LDS for a configured depth
 void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t stream);
-void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, hipStream_t stream);
+void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, int waves, hipStream_t stream);
 void launch_generate_bdpt(const GenArgs& a, const DevCapture& cap, hipStream_t stream);
 void launch_shade_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int grid, hipStream_t stream);
 void launch_resolve(const ResolveArgs& a, hipStream_t stream);
