@@ -160,6 +160,33 @@ int  yn_job_set_counting(yn_ctx* ctx, int32_t job_id, int32_t on);
 int  yn_job_capture(yn_ctx* ctx, int32_t job_id, int32_t* prim_tri, int32_t* prim_light,
                     float* prim_t, float* shadow_ray, int32_t* shadow_occ);
 
+/* ---- serving: continuous batching of many progressive jobs.  No
+ * counterpart in the reference (its RendererCore renders one image); the
+ * jobs keep the frame protocol of yn_job_render, so a job's image is the
+ * same whether it runs alone or among others. ----
+ * yn_submit queues a job of n_frames frames (spp); its accumulation image is
+ * admitted against cfg.mem_budget, oldest first, as memory frees up.
+ * yn_serve runs up to max_rounds scheduler rounds; each packs whole frames
+ * of the oldest admitted jobs (any scenes of the context) into one
+ * wavefront batch and folds them into the jobs' images.  *pending: served
+ * jobs not finished.  A finished job is read and released like any job. */
+int  yn_submit(yn_ctx* ctx, int32_t scene_id, const yn_cam* cam, int32_t width, int32_t height,
+               const yn_render_params* params, uint32_t base_seed, int32_t n_frames, int32_t* job_id);
+int  yn_serve(yn_ctx* ctx, int32_t max_rounds, int32_t* pending);
+typedef struct yn_job_status {
+    int32_t frames_target, frames_done, admitted, done;
+    double  t_submit_ms, t_start_ms, t_done_ms;   /* yn_clock_ms times; -1 = not yet */
+} yn_job_status;
+int  yn_job_status_get(yn_ctx* ctx, int32_t job_id, yn_job_status* out);
+int  yn_serve_stats(yn_ctx* ctx, yn_stats* out);     /* rays and kernel times of all rounds */
+double yn_clock_ms(yn_ctx* ctx);                     /* ms since yn_create */
+
+/* ---- diagnostics: the kernels' sin (op 0), cos (op 1) and pow (op 2) on
+ * the device for n host inputs (x, and y for pow).  OpenCL leaves these
+ * built-ins' exact results to the runtime; this library defines each once
+ * and evaluates it identically on host and device. ---- */
+int  yn_fmath_eval(yn_ctx* ctx, int32_t op, const float* x, const float* y, int32_t n, float* out);
+
 /* ---- traceRay (udpt.cl:240-286) on caller rays: n x 8 floats
  * (o.xyz, length, d.xyz, is_shadow).  Outputs per ray: triangle id (-1),
  * light id (-1 / 0), final length.  Rays with is_shadow != 0 take the

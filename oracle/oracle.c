@@ -9,6 +9,10 @@
  * source order.
  */
 #include "oracle.h"
+/* sin, cos and pow: OpenCL leaves their exact results to the runtime; the
+ * oracle uses the library's one definition of each (double evaluation,
+ * one rounding), pinned against libm in tests/test_fmath.py. */
+#include "../yune_amd/csrc/yn_fmath.h"
 
 #include <math.h>
 #include <float.h>
@@ -616,7 +620,7 @@ static v4 eval_brdf(const yn_material* m, v4 w_i, v4 w_o, v4 p, v4 n, int glossy
         if (on && rr_prob == 1.0f) return oren_nayar(m, w_i, w_o, p, n);
         return divs(muls(F4(m->kd), INV_PI), rr_prob);
     }
-    float cos_alpha = powf(fmaxf(dot(w_o, refl), 0.0f), m->px + m->py);
+    float cos_alpha = ynf_powf(fmaxf(dot(w_o, refl), 0.0f), m->px + m->py);
     int phong_exp = (int)(m->px + m->py);
     return divs(muls(muls(muls(muls(F4(m->ks), cos_alpha), (float)(phong_exp + 2)), INV_PI), 0.5f), rr_prob);
 }
@@ -665,17 +669,17 @@ static void phong_sample(const yn_material* m, ray_t* ray, float* pdf, v4 w_i, v
     float r2 = next_rand(seed);
     int phong_exponent = (int)(m->px + m->py);
     float phi = 2 * PI * r2;
-    float costheta = powf(r1, 1.0f / (float)(phong_exponent + 1));
-    float sintheta = 1 - powf(r1, 2.0f / (float)(phong_exponent + 1));
+    float costheta = ynf_powf(r1, 1.0f / (float)(phong_exponent + 1));
+    float sintheta = 1 - ynf_powf(r1, 2.0f / (float)(phong_exponent + 1));
     sintheta = sqrtf(sintheta);
-    float x = sintheta * cosf(phi), y = sintheta * sinf(phi), z = costheta;
+    float x = sintheta * ynf_cosf(phi), y = sintheta * ynf_sinf(phi), z = costheta;
     ray->d = to_world(Nx, Ny, Nz, p, V(x, y, z, 0));
     ray->o = add(p, muls(ray->d, EPSILON));
     ray->shadow = 0;
     ray->len = INFINITY;
     if (dot(ray->d, n) < 0) *pdf = 0;
     else *pdf = (float)((double)(phong_exponent + 1) * 0.5 * (double)INV_PI *
-                        (double)powf(costheta, (float)phong_exponent));
+                        (double)ynf_powf(costheta, (float)phong_exponent));
 }
 
 /* cosineWeightedHemisphere (udpt.cl:843-910) */
@@ -687,7 +691,7 @@ static void cosine_sample(ray_t* ray, float* pdf, v4 p, v4 n, uint32_t* seed)
     float r2 = next_rand(seed);
     float phi = 2 * PI * r2;
     float sinTheta = sqrtf(r1);
-    float x = sinTheta * cosf(phi), y = sinTheta * sinf(phi), z = sqrtf(1 - r1);
+    float x = sinTheta * ynf_cosf(phi), y = sinTheta * ynf_sinf(phi), z = sqrtf(1 - r1);
     ray->d = to_world(Nx, Ny, Nz, p, V(x, y, z, 0));
     ray->o = add(p, muls(ray->d, EPSILON));
     ray->shadow = 0;
@@ -721,8 +725,8 @@ static float fresnel(const yn_material* m, v4 w_i, v4 normal, float* ior_factor)
     float r0 = m->ks.x + m->ks.y + m->ks.z;
     r0 /= 3.0f;
     *ior_factor = (n2 * n2) / (n1 * n1);
-    if (n1 > n2) return (r0 + (1 - r0) * (1 - powf(cosThetaI, 5.0f)));
-    return (r0 + (1 - r0) * (1 - powf(cosThetaT, 5.0f)));
+    if (n1 > n2) return (r0 + (1 - r0) * (1 - ynf_powf(cosThetaI, 5.0f)));
+    return (r0 + (1 - r0) * (1 - ynf_powf(cosThetaT, 5.0f)));
 }
 
 /* sampleFresnelIncidence (udpt.cl:912-947) */
@@ -751,9 +755,9 @@ static float phong_pdf(const yn_material* m, v4 w_i, v4 w_o, v4 n)
 {
     v4 refl = sub(muls(n, 2 * (dot(w_o, n))), w_o);
     refl = normalize(refl);
-    float costheta = fmaxf(0.0f, cosf(dot(refl, w_i)));
+    float costheta = fmaxf(0.0f, ynf_cosf(dot(refl, w_i)));
     float pe = m->px + m->py;
-    return (float)((double)(pe + 1) * 0.5 * (double)INV_PI * (double)powf(costheta, pe));
+    return (float)((double)(pe + 1) * 0.5 * (double)INV_PI * (double)ynf_powf(costheta, pe));
 }
 
 static inline float cos_pdf(v4 w_i, v4 n) { return fmaxf(dot(w_i, n), 0.0f) * INV_PI; }   /* :1047 */
@@ -1251,10 +1255,10 @@ void or_tonemap(const float* in, int n, float* out)
         float lum_white = 1.0f;
         float lum_display = lum_world * (1 + lum_world / (lum_white * lum_white)) / (1 + lum_world);
         v4 sc = divs(col, lum_world);
-        v4 ldr = muls(V(powf(sc.x, 1.0f), powf(sc.y, 1.0f), powf(sc.z, 1.0f), powf(sc.w, 1.0f)), lum_display);
+        v4 ldr = muls(V(ynf_powf(sc.x, 1.0f), ynf_powf(sc.y, 1.0f), ynf_powf(sc.z, 1.0f), ynf_powf(sc.w, 1.0f)), lum_display);
         const float g = 1 / 2.2f;
-        out[4 * i] = powf(ldr.x, g); out[4 * i + 1] = powf(ldr.y, g);
-        out[4 * i + 2] = powf(ldr.z, g); out[4 * i + 3] = powf(ldr.w, g);
+        out[4 * i] = ynf_powf(ldr.x, g); out[4 * i + 1] = ynf_powf(ldr.y, g);
+        out[4 * i + 2] = ynf_powf(ldr.z, g); out[4 * i + 3] = ynf_powf(ldr.w, g);
     }
 }
 
@@ -1338,3 +1342,8 @@ float or_phong_pdf(const yn_material* m, const float n[4], const float w_i[4], c
 {
     return phong_pdf(m, A4(w_i), A4(w_o), A4(n));
 }
+
+/* the shared sin / cos / pow, exported for tests/test_fmath.py */
+float or_fm_sinf(float x) { return ynf_sinf(x); }
+float or_fm_cosf(float x) { return ynf_cosf(x); }
+float or_fm_powf(float x, float y) { return ynf_powf(x, y); }

@@ -121,6 +121,9 @@ int  or_ray_box(const float o[4], const float d[4], const yn_aabb* bb);
 void or_primary_ray(const yn_cam* cam, int W, int H, uint32_t rand, int x, int y, float o[4], float d[4]);
 float or_phong_pdf(const yn_material* m, const float n[4], const float w_i[4], const float w_o[4]);
 int  or_ray_tri(const yn_triangle* tri, const float o[4], const float d[4], float* len);
+float or_fm_sinf(float x);
+float or_fm_cosf(float x);
+float or_fm_powf(float x, float y);
 
 #ifdef __cplusplus
 }

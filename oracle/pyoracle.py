@@ -114,6 +114,11 @@ def lib():
                                      C.c_void_p]
         L.or_phong_pdf.restype = C.c_float
         L.or_phong_pdf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        for fn in ("or_fm_sinf", "or_fm_cosf"):
+            getattr(L, fn).restype = C.c_float
+            getattr(L, fn).argtypes = [C.c_float]
+        L.or_fm_powf.restype = C.c_float
+        L.or_fm_powf.argtypes = [C.c_float, C.c_float]
         L.or_ray_tri.restype = C.c_int
         L.or_ray_tri.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
         _o = L

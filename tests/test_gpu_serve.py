@@ -1,0 +1,110 @@
+"""GPU tests of the kernel-contract entry point and of the scheduler.
+
+* yn_pathtracer (the reference kernel's 14 arguments: tiles, reset, rand,
+  ping-pong images) gives the same image as yn_job_render with the same
+  frame seeds, bit for bit;
+* jobs served together (several scenes and sizes in one wavefront batch)
+  give the same image as each job rendered alone, bit for bit: a path's
+  arithmetic depends only on its pixel, frame seed and scene, and each
+  pixel's frames are folded in frame order either way;
+* admission against the memory budget holds jobs back until images are
+  released, and they still finish with the same result.
+"""
+import numpy as np
+import pytest
+
+import yune_amd as Y
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_kernel_contract_tiles_and_ping_pong_match_job():
+    import torch
+    s = Y.gen_cornell()
+    W, H = 70, 50                     # tiles of uneven size (ceil(W/2), ceil(H/2))
+    p = Y.params_reference(mis=1, max_bounces=8, rr_threshold=2)
+    seeds = O.mt19937(9, 2)
+    with Y.Context(max_paths=1 << 16) as c:
+        sid = c.scene(s)
+        a = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+        b = torch.zeros_like(a)
+        for blk in range(4):          # frame 0: reset, into a
+            c.pathtracer(a, None, s.cam, sid, W, H, p, 1, 1, int(seeds[0]), blk, 2, 2)
+        for blk in range(4):          # frame 1: read a, write b (ping-pong)
+            c.pathtracer(b, a, s.cam, sid, W, H, p, 1, 0, int(seeds[1]), blk, 2, 2)
+        torch.cuda.synchronize()
+        got = b.cpu().numpy()
+        job = c.job(sid, s.cam, W, H, p, base_seed=9)
+        job.render(2)
+        ref = job.read()
+    assert np.array_equal(got.view(np.int32), ref.view(np.int32))
+    assert (got[..., 3] == 2).all()
+
+
+def _jobs(scenes):
+    rng = np.random.default_rng(4)
+    out = []
+    for k in range(7):
+        sid = int(rng.integers(len(scenes)))
+        W, H = [(48, 32), (64, 64), (33, 21), (96, 40)][k % 4]
+        mode = Y.YN_MODE_BDPT if k == 5 else Y.YN_MODE_PT
+        kw = dict(bdpt_bounces=4) if mode else dict(mis=k % 2, max_bounces=8, rr_threshold=2)
+        out.append(dict(sid=sid, W=W, H=H, p=Y.params_reference(mode, **kw), frames=int(rng.integers(2, 9)),
+                        seed=int(rng.integers(1, 1 << 30))))
+    return out
+
+
+def test_served_jobs_equal_jobs_alone():
+    scenes = [Y.gen_cornell(), Y.gen_icosphere_room(3, 9, 2)]
+    with Y.Context(max_paths=1 << 15) as c:
+        sids = [c.scene(s) for s in scenes]
+        specs = _jobs(scenes)
+        jobs = [c.submit(sids[j["sid"]], scenes[j["sid"]].cam, j["W"], j["H"], j["p"], j["frames"], j["seed"])
+                for j in specs]
+        for _ in range(1000):
+            if c.serve(4) == 0:
+                break
+        served = []
+        for jb, j in zip(jobs, specs):
+            st = jb.status()
+            assert st["done"] == 1 and st["frames_done"] == j["frames"]
+            assert st["t_done_ms"] >= st["t_start_ms"] >= st["t_submit_ms"] >= 0
+            served.append(jb.read())
+            jb.release()
+        for img, j in zip(served, specs):
+            alone = c.job(sids[j["sid"]], scenes[j["sid"]].cam, j["W"], j["H"], j["p"], base_seed=j["seed"])
+            alone.render(j["frames"])
+            ref = alone.read()
+            alone.release()
+            assert np.array_equal(img.view(np.int32), ref.view(np.int32))
+        assert c.serve_stats().rays_ext > 0
+
+
+def test_admission_waits_for_memory():
+    s = Y.gen_cornell()
+    W, H = 64, 64
+    img_bytes = W * H * 16
+    p = Y.params_reference(mis=1, max_bounces=8, rr_threshold=2)
+    with Y.Context(max_paths=1 << 15, mem_budget=2 * img_bytes + 100) as c:
+        sid = c.scene(s)
+        jobs = [c.submit(sid, s.cam, W, H, p, 3, base_seed=10 + k) for k in range(4)]
+        st = [j.status() for j in jobs]
+        assert [x["admitted"] for x in st] == [1, 1, 0, 0]
+        with pytest.raises(Y.YuneError):
+            jobs[3].read()
+        done = {}
+        for _ in range(200):
+            c.serve(1)
+            for k, j in enumerate(jobs):
+                if k not in done and j.status()["done"]:
+                    done[k] = j.read()
+                    j.release()
+            if len(done) == 4:
+                break
+        assert len(done) == 4
+        for k in range(4):
+            alone = c.job(sid, s.cam, W, H, p, base_seed=10 + k)
+            alone.render(3)
+            assert np.array_equal(done[k].view(np.int32), alone.read().view(np.int32))
+            alone.release()

@@ -10,6 +10,7 @@ Python mirror of the reference's host side for tests and benchmarks:
   Context.job / Job  RendererCore frame loop   (src/RendererCore.cpp:248-579)
   Context.pathtracer the 14-argument kernel    (kernels/legacy/udpt.cl:158-160)
   Context.trace      traceRay                  (kernels/legacy/udpt.cl:240-286)
+  Context.submit / serve  many jobs per wavefront batch (no reference counterpart)
 
 All work is done by libyune_hip.so; nothing here computes a result itself.
 """
@@ -20,7 +21,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import (MAT_DTYPE, NODE_DTYPE, TRI_DTYPE, YN_MODE_BDPT, YN_MODE_PT, Cam, Config, HostScene,
+from ._lib import (MAT_DTYPE, NODE_DTYPE, TRI_DTYPE, YN_MODE_BDPT, YN_MODE_PT, Cam, Config, HostScene, JobStatus,
                    LightQuad, RenderParams, SceneInfo, Stats, YuneError, check, lib, ptr)
 
 __all__ = ["TRI_DTYPE", "MAT_DTYPE", "NODE_DTYPE", "YN_MODE_PT", "YN_MODE_BDPT", "YuneError", "Scene",
@@ -183,6 +184,43 @@ class Context:
                                      C.byref(jid)))
         return Job(self, jid.value, width, height)
 
+    def pathtracer(self, out_img, in_img, cam: Cam, sid: int, width: int, height: int, params: RenderParams,
+                   gi_check: int, reset: int, rand: int, block: int = 0, block_x: int = 1, block_y: int = 1):
+        """One frame over tile `block` of a block_x x block_y grid on device
+        images (torch float32 tensors of H x W x 4 on the GPU), with the
+        meaning of the reference kernel's arguments 0-13."""
+        import torch
+        for t in (out_img,) + ((in_img,) if in_img is not None else ()):
+            if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.numel() == width * height * 4):
+                raise ValueError("images must be contiguous float32 CUDA tensors of height x width x 4")
+        torch.cuda.synchronize()            # the library works on its own stream
+        self._ck(lib().yn_pathtracer(self._p, C.c_void_p(out_img.data_ptr()),
+                                     C.c_void_p(in_img.data_ptr()) if in_img is not None else None,
+                                     C.byref(cam), sid, width, height, C.byref(params), gi_check, reset, rand,
+                                     block, block_x, block_y))
+
+    def submit(self, sid: int, cam: Cam, width: int, height: int, params: RenderParams, frames: int,
+               base_seed: int = 1) -> "Job":
+        """Queue a job of `frames` frames for the scheduler (yn_submit)."""
+        jid = C.c_int32()
+        self._ck(lib().yn_submit(self._p, sid, C.byref(cam), width, height, C.byref(params), base_seed, frames,
+                                 C.byref(jid)))
+        return Job(self, jid.value, width, height)
+
+    def serve(self, max_rounds: int = 1) -> int:
+        """Run up to max_rounds scheduler rounds; returns the jobs still pending."""
+        pend = C.c_int32()
+        self._ck(lib().yn_serve(self._p, max_rounds, C.byref(pend)))
+        return pend.value
+
+    def serve_stats(self) -> Stats:
+        st = Stats()
+        self._ck(lib().yn_serve_stats(self._p, C.byref(st)))
+        return st
+
+    def clock_ms(self) -> float:
+        return lib().yn_clock_ms(self._p)
+
     def trace(self, sid: int, rays: np.ndarray):
         """rays: (n, 8) float32 (o.xyz, length, d.xyz, is_shadow)."""
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
@@ -227,6 +265,11 @@ class Job:
         st = Stats()
         self.ctx._ck(lib().yn_job_stats(self.ctx._p, self.id, C.byref(st)))
         return st
+
+    def status(self) -> dict:
+        st = JobStatus()
+        self.ctx._ck(lib().yn_job_status_get(self.ctx._p, self.id, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in JobStatus._fields_}
 
     def set_counting(self, on: bool):
         self.ctx._ck(lib().yn_job_set_counting(self.ctx._p, self.id, int(on)))

@@ -67,6 +67,12 @@ class SceneInfo(C.Structure):
                 ("tri_bytes", C.c_size_t), ("shade_bytes", C.c_size_t)]
 
 
+class JobStatus(C.Structure):
+    _fields_ = [("frames_target", C.c_int32), ("frames_done", C.c_int32), ("admitted", C.c_int32),
+                ("done", C.c_int32), ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double),
+                ("t_done_ms", C.c_double)]
+
+
 class HostScene(C.Structure):
     # yn_light_quad is 16-byte aligned in C: explicit pad before it
     _fields_ = [("tris", C.c_void_p), ("n_tri", C.c_int32), ("mats", C.c_void_p), ("n_mat", C.c_int32),
@@ -114,6 +120,12 @@ SIGNATURES = {
     "yn_job_set_counting": (I32, [P, I32, I32]),
     "yn_job_capture": (I32, [P, I32, P, P, P, P, P]),
     "yn_trace": (I32, [P, I32, P, I32, P, P, P]),
+    "yn_submit": (I32, [P, I32, C.POINTER(Cam), I32, I32, C.POINTER(RenderParams), C.c_uint32, I32, PI32]),
+    "yn_serve": (I32, [P, I32, PI32]),
+    "yn_job_status_get": (I32, [P, I32, C.POINTER(JobStatus)]),
+    "yn_serve_stats": (I32, [P, C.POINTER(Stats)]),
+    "yn_clock_ms": (C.c_double, [P]),
+    "yn_fmath_eval": (I32, [P, I32, P, P, I32, P]),
 }
 
 _lib = None

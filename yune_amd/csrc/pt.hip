@@ -454,9 +454,9 @@ __global__ __launch_bounds__(256) void k_tonemap(const float4* in, float4* out, 
     float lum_white = 1.0f;
     float lum_display = lum_world * (1 + lum_world / (lum_white * lum_white)) / (1 + lum_world);
     f4 sc = col / lum_world;
-    f4 ldr = mk(powf(sc.x, 1.0f), powf(sc.y, 1.0f), powf(sc.z, 1.0f), powf(sc.w, 1.0f)) * lum_display;
+    f4 ldr = mk(ynf_powf(sc.x, 1.0f), ynf_powf(sc.y, 1.0f), ynf_powf(sc.z, 1.0f), ynf_powf(sc.w, 1.0f)) * lum_display;
     const float g = 1 / 2.2f;
-    out[i] = make_float4(powf(ldr.x, g), powf(ldr.y, g), powf(ldr.z, g), powf(ldr.w, g));
+    out[i] = make_float4(ynf_powf(ldr.x, g), ynf_powf(ldr.y, g), ynf_powf(ldr.z, g), ynf_powf(ldr.w, g));
 }
 
 // Rays given by a caller (n x 8 floats: o, len, d, is_shadow): quad test
@@ -476,6 +476,19 @@ __global__ __launch_bounds__(256) void k_prepare_rays(const DevScene* scene, con
     q.o_t[i] = make_float4(o.x, o.y, o.z, len);
     q.d_p[i] = make_float4(d.x, d.y, d.z, __int_as_float(i));
     if (i == 0) *q.count = (uint32_t)n;
+}
+
+// The kernels' sin / cos / pow evaluated on the device (diagnostics).
+__global__ __launch_bounds__(256) void k_fmath(int op, const float* x, const float* y, int n, float* out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = op == 0 ? ynf_sinf(x[i]) : op == 1 ? ynf_cosf(x[i]) : ynf_powf(x[i], y[i]);
+}
+
+void launch_fmath(int op, const float* x, const float* y, int n, float* out, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fmath, dim3((n + 255) / 256), dim3(256), 0, s, op, x, y, n, out);
 }
 
 void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t s)

@@ -58,6 +58,13 @@ struct JobRec {
     float* cap_prim_t = nullptr;
     float* cap_shadow = nullptr;
     int32_t* cap_shadow_occ = nullptr;
+    // serving (yn_submit / yn_serve)
+    bool served = false;          // submitted for the scheduler
+    bool admitted = true;         // accumulation image allocated
+    int64_t target = 0;           // frames requested
+    int64_t seq = 0;              // submission order
+    size_t bytes = 0;             // accumulation image bytes held
+    double t_submit = 0.0, t_start = -1.0, t_done = -1.0;   // ms since context creation
 };
 
 constexpr int kMaxScenes = 64;
@@ -104,6 +111,10 @@ struct yn_ctx {
     uint16_t* ovf_t = nullptr;
     int ovf_depth = 0;
     int64_t ovf_threads = 0;
+    // serving
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    int64_t submit_seq = 0;
+    JobRec serve_rec;                 // statistics of scheduler rounds
 };
 
 namespace {
@@ -526,6 +537,113 @@ int render_rect(yn_ctx* c, int jid, float4* accum, int x0, int y0, int w, int h,
     return 0;
 }
 
+double ms_since(const yn_ctx* c)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c->t0).count();
+}
+
+// Admission: served jobs get their accumulation image, oldest first, while
+// the memory budget allows (a job that does not fit waits; younger jobs that
+// fit may go ahead of it).
+int admit_jobs(yn_ctx* c)
+{
+    std::vector<int> wait;
+    for (int i = 0; i < (int)c->jobs.size(); ++i)
+        if (c->jobs[i].live && c->jobs[i].served && !c->jobs[i].admitted) wait.push_back(i);
+    std::sort(wait.begin(), wait.end(), [&](int a, int b) { return c->jobs[a].seq < c->jobs[b].seq; });
+    for (int jid : wait) {
+        JobRec& J = c->jobs[jid];
+        size_t bytes = (size_t)J.W * J.H * sizeof(float4);
+        if (c->job_bytes + bytes > c->cfg.mem_budget) continue;
+        int r = dalloc(c, &J.accum, (size_t)J.W * J.H);
+        if (r) return r;
+        J.bytes = bytes;
+        J.admitted = true;
+        c->job_bytes += bytes;
+    }
+    return 0;
+}
+
+// One scheduler round: whole frames of admitted jobs, oldest job first, are
+// packed into one wavefront batch (jobs of one mode per round; a job's
+// frames stay in order), run, and folded into each job's image.  Returns the
+// number of frames rendered (0: nothing to do).
+int serve_round(yn_ctx* c, int* frames_out)
+{
+    *frames_out = 0;
+    if (int r = admit_jobs(c)) return r;
+    std::vector<int> order;
+    int max_kb = 0;
+    for (int i = 0; i < (int)c->jobs.size(); ++i) {
+        const JobRec& J = c->jobs[i];
+        if (J.live && J.served && J.admitted && (int64_t)J.frame < J.target) {
+            order.push_back(i);
+            if (J.p.mode == YN_MODE_BDPT) max_kb = std::max(max_kb, J.p.bdpt_bounces);
+        }
+    }
+    if (order.empty()) return 0;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return c->jobs[a].seq < c->jobs[b].seq; });
+    const int mode = c->jobs[order[0]].p.mode;
+    int64_t pcap = c->cap_paths;
+    if (mode == YN_MODE_BDPT) {
+        if (int r = ensure_bdpt(c, max_kb)) return r;
+        pcap = c->cap_bdpt;
+    }
+    // a job whose frame exceeds the batch runs alone, in row bands
+    {
+        JobRec& J = c->jobs[order[0]];
+        if ((int64_t)J.W * J.H > pcap) {
+            if (J.t_start < 0) J.t_start = ms_since(c);
+            std::vector<uint32_t> seeds(1, (uint32_t)J.engine());
+            int r = render_rect(c, order[0], J.accum, 0, 0, J.W, J.H, seeds, J.frame == 0, DevCapture{});
+            if (r) return r;
+            HIPCK(c, hipStreamSynchronize(c->stream));
+            J.frame += 1;
+            J.st.frames += 1;
+            J.st.samples += (uint64_t)J.W * J.H;
+            if ((int64_t)J.frame >= J.target) J.t_done = ms_since(c);
+            *frames_out = 1;
+            return 0;
+        }
+    }
+    struct Plan { int jid; int k; int64_t off; bool reset; };
+    std::vector<Plan> plan;
+    std::vector<Region> regs;
+    int64_t off = 0;
+    for (int jid : order) {
+        JobRec& J = c->jobs[jid];
+        if (J.p.mode != mode) continue;
+        int64_t px = (int64_t)J.W * J.H;
+        int64_t k = std::min<int64_t>(J.target - (int64_t)J.frame, (pcap - off) / px);
+        if (k <= 0) continue;
+        for (int64_t f = 0; f < k; ++f) regs.push_back({jid, (int32_t)f, (uint32_t)J.engine(), 0, 0, J.W, J.H});
+        plan.push_back({jid, (int)k, off, J.frame == 0});
+        off += k * px;
+        if (J.t_start < 0) J.t_start = ms_since(c);
+    }
+    if (regs.empty()) return 0;
+    for (const Plan& pl : plan)
+        if (int r = upload_job_table(c, pl.jid, c->samples + pl.off)) return r;
+    if (int r = run_batch(c, regs, c->serve_rec, DevCapture{})) return r;
+    for (const Plan& pl : plan) {
+        JobRec& J = c->jobs[pl.jid];
+        ResolveArgs ra{c->d_jobs, pl.jid, 0, 0, J.W, J.H, pl.k, pl.reset ? 1 : 0, J.accum};
+        launch_resolve(ra, c->stream);
+    }
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    double now = ms_since(c);
+    for (const Plan& pl : plan) {
+        JobRec& J = c->jobs[pl.jid];
+        J.frame += (uint64_t)pl.k;
+        J.st.frames += (uint64_t)pl.k;
+        J.st.samples += (uint64_t)pl.k * J.W * J.H;
+        if ((int64_t)J.frame >= J.target) J.t_done = now;
+        *frames_out += pl.k;
+    }
+    return 0;
+}
+
 int check_ctx(yn_ctx* c) { return c ? 0 : fail(nullptr, YN_ERR_INVALID, "null context"); }
 int check_job(yn_ctx* c, int jid)
 {
@@ -753,6 +871,7 @@ int yn_job_create(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t 
     J.H = H;
     J.p = *p;
     J.engine.seed(base_seed);
+    J.bytes = bytes;
     c->job_bytes += bytes;
     *job_id = jid;
     return upload_job_table(c, jid, c->samples);
@@ -761,6 +880,7 @@ int yn_job_create(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t 
 int yn_job_render(yn_ctx* c, int32_t jid, int32_t n_frames)
 {
     if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (!c->jobs[jid].admitted) return fail(c, YN_ERR_INVALID, "job not admitted yet (memory budget)");
     if (n_frames < 0) return fail(c, YN_ERR_INVALID, "negative frame count");
     JobRec& J = c->jobs[jid];
     auto t0 = std::chrono::steady_clock::now();
@@ -836,6 +956,7 @@ int yn_job_stats(yn_ctx* c, int32_t jid, yn_stats* out)
 int yn_job_read(yn_ctx* c, int32_t jid, float* rgba)
 {
     if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (!c->jobs[jid].admitted) return fail(c, YN_ERR_INVALID, "job not admitted yet (memory budget)");
     if (!rgba) return fail(c, YN_ERR_INVALID, "null output");
     JobRec& J = c->jobs[jid];
     HIPCK(c, hipStreamSynchronize(c->stream));
@@ -846,6 +967,7 @@ int yn_job_read(yn_ctx* c, int32_t jid, float* rgba)
 int yn_job_read_tonemapped(yn_ctx* c, int32_t jid, float* rgba)
 {
     if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (!c->jobs[jid].admitted) return fail(c, YN_ERR_INVALID, "job not admitted yet (memory budget)");
     if (!rgba) return fail(c, YN_ERR_INVALID, "null output");
     JobRec& J = c->jobs[jid];
     size_t n = (size_t)J.W * J.H;
@@ -863,6 +985,7 @@ int yn_job_read_tonemapped(yn_ctx* c, int32_t jid, float* rgba)
 int yn_job_save(yn_ctx* c, int32_t jid, const char* path)
 {
     if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (!c->jobs[jid].admitted) return fail(c, YN_ERR_INVALID, "job not admitted yet (memory budget)");
     if (!path) return fail(c, YN_ERR_INVALID, "null path");
     JobRec& J = c->jobs[jid];
     std::string p(path);
@@ -900,8 +1023,104 @@ int yn_job_release(yn_ctx* c, int32_t jid)
     JobRec& J = c->jobs[jid];
     HIPCK(c, hipStreamSynchronize(c->stream));
     if (J.accum) hipFree(J.accum);
-    c->job_bytes -= (size_t)J.W * J.H * sizeof(float4);
+    c->job_bytes -= J.bytes;
     J = JobRec();
+    return 0;
+}
+
+int yn_submit(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H, const yn_render_params* p,
+              uint32_t base_seed, int32_t n_frames, int32_t* job_id)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (check_scene(c, sid)) return YN_ERR_NOT_FOUND;
+    if (!valid_cam(cam)) return fail(c, YN_ERR_INVALID, "camera row 4 must be (0,0,0,1)");
+    if (!valid_params(p)) return fail(c, YN_ERR_INVALID, "bad render parameters");
+    if (W <= 0 || H <= 0 || (int64_t)W * H > (1ll << 28) || !job_id || n_frames < 0)
+        return fail(c, YN_ERR_INVALID, "bad image size or frame count");
+    if ((size_t)W * H * sizeof(float4) > c->cfg.mem_budget)
+        return fail(c, YN_ERR_NO_MEMORY, "image larger than the job memory budget");
+    int jid = -1;
+    for (int i = 0; i < kMaxJobs; ++i) if (!c->jobs[i].live && i != c->pt_job_slot) { jid = i; break; }
+    if (jid < 0) return fail(c, YN_ERR_NO_MEMORY, "too many jobs");
+    JobRec& J = c->jobs[jid];
+    J = JobRec();
+    J.live = true;
+    J.served = true;
+    J.admitted = false;
+    J.scene = sid;
+    J.cam = *cam;
+    J.W = W;
+    J.H = H;
+    J.p = *p;
+    J.engine.seed(base_seed);
+    J.target = n_frames;
+    J.seq = c->submit_seq++;
+    J.t_submit = ms_since(c);
+    if (n_frames == 0) J.t_done = J.t_submit;
+    *job_id = jid;
+    return admit_jobs(c);
+}
+
+int yn_serve(yn_ctx* c, int32_t max_rounds, int32_t* pending)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    for (int r = 0; r < max_rounds; ++r) {
+        int frames = 0;
+        if (int e = serve_round(c, &frames)) return e;
+        if (frames == 0) break;
+    }
+    if (pending) {
+        int n = 0;
+        for (const JobRec& J : c->jobs) n += (J.live && J.served && (int64_t)J.frame < J.target) ? 1 : 0;
+        *pending = n;
+    }
+    return 0;
+}
+
+int yn_job_status_get(yn_ctx* c, int32_t jid, yn_job_status* out)
+{
+    if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (!out) return fail(c, YN_ERR_INVALID, "null output");
+    const JobRec& J = c->jobs[jid];
+    out->frames_target = (int32_t)(J.served ? J.target : (int64_t)J.frame);
+    out->frames_done = (int32_t)J.frame;
+    out->admitted = J.admitted ? 1 : 0;
+    out->done = (J.served && (int64_t)J.frame >= J.target) ? 1 : 0;
+    out->t_submit_ms = J.t_submit;
+    out->t_start_ms = J.t_start;
+    out->t_done_ms = J.t_done;
+    return 0;
+}
+
+int yn_serve_stats(yn_ctx* c, yn_stats* out)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (!out) return fail(c, YN_ERR_INVALID, "null output");
+    *out = c->serve_rec.st;
+    return 0;
+}
+
+double yn_clock_ms(yn_ctx* c) { return c ? ms_since(c) : 0.0; }
+
+int yn_fmath_eval(yn_ctx* c, int32_t op, const float* x, const float* y, int32_t n, float* out)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (op < 0 || op > 2 || n < 0 || (n > 0 && (!x || !out || (op == 2 && !y))))
+        return fail(c, YN_ERR_INVALID, "bad fmath arguments");
+    if (n == 0) return 0;
+    float* d = nullptr;
+    if (int r = dalloc(c, &d, (size_t)n * 3)) return r;
+    hipStream_t s = c->stream;
+    hipError_t e = hipMemcpyAsync(d, x, (size_t)n * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && op == 2) e = hipMemcpyAsync(d + n, y, (size_t)n * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        launch_fmath(op, d, d + n, n, d + 2 * (size_t)n, s);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * (size_t)n, (size_t)n * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(d);
+    if (e != hipSuccess) return fail(c, YN_ERR_HIP, std::string("fmath: ") + hipGetErrorString(e));
     return 0;
 }
 

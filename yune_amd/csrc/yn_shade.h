@@ -8,6 +8,7 @@
 
 #include "yn_device.h"
 #include "yn_math.h"
+#include "yn_fmath.h"
 
 namespace yune {
 
@@ -107,7 +108,7 @@ __device__ __forceinline__ f4 eval_brdf(const Mat& m, f4 w_i, f4 w_o, f4 p, f4 n
         if (on && rr_prob == 1.0f) return oren_nayar(m, w_i, w_o, p, n);
         return (m.kd * kInvPI) / rr_prob;
     }
-    float cos_alpha = powf(fmaxf(dot(w_o, refl), 0.0f), m.px + m.py);
+    float cos_alpha = ynf_powf(fmaxf(dot(w_o, refl), 0.0f), m.px + m.py);
     int phong_exp = (int)(m.px + m.py);
     return ((((m.ks * cos_alpha) * (float)(phong_exp + 2)) * kInvPI) * 0.5f) / rr_prob;
 }
@@ -145,12 +146,12 @@ __device__ __forceinline__ f4 phong_sample(const Mat& m, float& pdf, f4 w_i, f4 
     float r2 = next_rand(seed);
     int pe = (int)(m.px + m.py);
     float phi = 2 * kPI * r2;
-    float costheta = powf(r1, 1.0f / (float)(pe + 1));
-    float sintheta = 1 - powf(r1, 2.0f / (float)(pe + 1));
+    float costheta = ynf_powf(r1, 1.0f / (float)(pe + 1));
+    float sintheta = 1 - ynf_powf(r1, 2.0f / (float)(pe + 1));
     sintheta = sqrtf(sintheta);
-    f4 d = to_world(Nx, Ny, Nz, p, mk(sintheta * cosf(phi), sintheta * sinf(phi), costheta, 0));
+    f4 d = to_world(Nx, Ny, Nz, p, mk(sintheta * ynf_cosf(phi), sintheta * ynf_sinf(phi), costheta, 0));
     if (dot(d, n) < 0) pdf = 0;
-    else pdf = (float)((double)(pe + 1) * 0.5 * (double)kInvPI * (double)powf(costheta, (float)pe));
+    else pdf = (float)((double)(pe + 1) * 0.5 * (double)kInvPI * (double)ynf_powf(costheta, (float)pe));
     return d;
 }
 
@@ -164,7 +165,7 @@ __device__ __forceinline__ f4 cosine_sample(float& pdf, f4 p, f4 n, uint32_t& se
     float phi = 2 * kPI * r2;
     float sinTheta = sqrtf(r1);
     float z = sqrtf(1 - r1);
-    f4 d = to_world(Nx, Ny, n, p, mk(sinTheta * cosf(phi), sinTheta * sinf(phi), z, 0));
+    f4 d = to_world(Nx, Ny, n, p, mk(sinTheta * ynf_cosf(phi), sinTheta * ynf_sinf(phi), z, 0));
     pdf = z * kInvPI;
     return d;
 }
@@ -195,8 +196,8 @@ __device__ __forceinline__ float fresnel(const Mat& m, f4 w_i, f4 normal, float&
     float r0 = m.ks.x + m.ks.y + m.ks.z;
     r0 /= 3.0f;
     ior_factor = (n2 * n2) / (n1 * n1);
-    if (n1 > n2) return (r0 + (1 - r0) * (1 - powf(cI, 5.0f)));
-    return (r0 + (1 - r0) * (1 - powf(cT, 5.0f)));
+    if (n1 > n2) return (r0 + (1 - r0) * (1 - ynf_powf(cI, 5.0f)));
+    return (r0 + (1 - r0) * (1 - ynf_powf(cT, 5.0f)));
 }
 
 // sampleFresnelIncidence (udpt.cl:912-947): returns the new direction.
@@ -217,9 +218,9 @@ __device__ __forceinline__ f4 fresnel_sample(const Mat& m, f4 n, f4 w_i, float& 
 __device__ __forceinline__ float phong_pdf(const Mat& m, f4 w_i, f4 w_o, f4 n)
 {
     f4 refl = normalize(n * (2 * (dot(w_o, n))) - w_o);
-    float costheta = fmaxf(0.0f, cosf(dot(refl, w_i)));
+    float costheta = fmaxf(0.0f, ynf_cosf(dot(refl, w_i)));
     float pe = m.px + m.py;
-    return (float)((double)(pe + 1) * 0.5 * (double)kInvPI * (double)powf(costheta, pe));
+    return (float)((double)(pe + 1) * 0.5 * (double)kInvPI * (double)ynf_powf(costheta, pe));
 }
 __device__ __forceinline__ float cos_pdf(f4 w_i, f4 n) { return fmaxf(dot(w_i, n), 0.0f) * kInvPI; }
 __device__ __forceinline__ float power_heuristic(float w, float p1, float p2) { return (w * w) / (p1 * p1 + p2 * p2); }
