@@ -14,6 +14,11 @@ Other single-GPU workloads (--workload):
   bdpt     bidirectional (bdpt.cl semantics) on the Cornell box, 1280x720,
            5-vertex subpaths, MIS; 4 frames per step, 8 steps = 32 spp
 
+  batch    64 jobs on 8 icosphere scenes (20k-2M triangles), random cameras,
+           128^2-1024^2, 16-256 spp, all submitted at once to the scheduler
+  serve    the same job mix arriving as a Poisson stream (--rate, --duration);
+           reports jobs/s, p50/p99 latency and the plain per-job paths
+
 Rays counted: every BVH query the GPU performs (extension, NEE shadow, MIS
 any-hit, BDPT connection); the CPU baseline counts with the same rule.
 
@@ -51,6 +56,131 @@ WORKLOADS = {
         desc="bidirectional path tracing (bdpt.cl), Cornell box ({n} tris), 1280x720, {f} spp per step, "
              "5-vertex eye and light subpaths, all connections, NEE+MIS"),
 }
+
+
+def serving_scenes(Y, np, seed=77):
+    """8 icosphere rooms with triangle counts log-uniform over 20k-2M
+    (ceil(N / 20480) spheres of 20,480 triangles each)."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(2e4), np.log(2e6), 8))
+    return [Y.gen_icosphere_room(100 + i, int(np.ceil(n / 20480)), 5) for i, n in enumerate(counts)]
+
+
+def serving_jobs(Y, np, n_scenes, n, rate, seed=78):
+    """Jobs as in the batched / served configurations: a scene, a random
+    camera inside the room, resolution from {128, 256, 512, 1024}^2, spp
+    log-uniform over 16-256, depth 8, RR from bounce 3, NEE+MIS; Poisson
+    arrivals at `rate` per second (rate 0: all at t = 0)."""
+    rng = np.random.default_rng(seed)
+    params = Y.params_reference(mis=1, max_bounces=8, rr_threshold=2)
+    t = 0.0
+    jobs = []
+    for k in range(n):
+        if rate > 0:
+            t += rng.exponential(1.0 / rate)
+        eye = rng.uniform(-4, 4, 3)
+        look = rng.uniform(-4, 4, 3) - eye
+        look /= np.linalg.norm(look)
+        side = np.cross(look, (0.0, 1.0, 0.0))
+        side /= np.linalg.norm(side)
+        up = np.cross(side, look)
+        res = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(round(np.exp(rng.uniform(np.log(16), np.log(256)))))
+        jobs.append(dict(t=t, scene=int(rng.integers(n_scenes)), cam=Y.camera(side, up, look, eye, 60.0), W=res, H=res,
+                         spp=spp, seed=int(rng.integers(1, 1 << 30)), params=params))
+    return jobs
+
+
+def serve_bench(args, Y, np):
+    """Served (continuous batching through the scheduler) vs one job at a
+    time on the plain per-job path, on the same jobs (so equal spp and
+    estimator: the images are identical)."""
+    serve = args.workload == "serve"
+    scenes = serving_scenes(Y, np)
+    n_jobs = args.jobs or (int(args.duration * args.rate) if serve else 64)
+    jobs = serving_jobs(Y, np, len(scenes), n_jobs, args.rate if serve else 0.0)
+    lat, done_t = [], []
+    with Y.Context(mem_budget=64 << 30) as ctx:
+        t0 = time.perf_counter()
+        sids = [ctx.scene(s) for s in scenes]
+        print(f"# {len(scenes)} scenes, {sum(len(s.tris) for s in scenes)} triangles, uploaded in "
+              f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        # warm the kernels once
+        w = ctx.job(sids[0], jobs[0]["cam"], 64, 64, jobs[0]["params"])
+        w.render(1)
+        w.release()
+        out = {}
+        clock0 = ctx.clock_ms()
+        nxt = 0
+        live = {}
+        while nxt < len(jobs) or live:
+            now = ctx.clock_ms() - clock0
+            while nxt < len(jobs) and jobs[nxt]["t"] * 1e3 <= now:
+                j = jobs[nxt]
+                live[nxt] = ctx.submit(sids[j["scene"]], j["cam"], j["W"], j["H"], j["params"], j["spp"], j["seed"])
+                nxt += 1
+            if live:
+                ctx.serve(1)
+            elif nxt < len(jobs):
+                time.sleep(max(0.0, jobs[nxt]["t"] - now / 1e3))
+            for k in list(live):
+                st = live[k].status()
+                if st["done"]:
+                    lat.append(st["t_done_ms"] - clock0 - jobs[k]["t"] * 1e3)
+                    done_t.append(st["t_done_ms"] - clock0)
+                    if k < 3:
+                        out[k] = live[k].read()
+                    live[k].release()
+                    del live[k]
+        makespan = max(done_t) / 1e3
+        sst = ctx.serve_stats()
+        # plain paths on a sample of the same jobs
+        n_plain = min(len(jobs), args.plain_jobs)
+        t_frame = t_batch = 0.0
+        same = True
+        for k in range(n_plain):
+            j = jobs[k]
+            a = ctx.job(sids[j["scene"]], j["cam"], j["W"], j["H"], j["params"], base_seed=j["seed"])
+            t1 = time.perf_counter()
+            for _ in range(j["spp"]):
+                a.render(1)
+            t_frame += time.perf_counter() - t1
+            if k in out:
+                same &= bool(np.array_equal(a.read().view(np.int32), out[k].view(np.int32)))
+            a.release()
+            b = ctx.job(sids[j["scene"]], j["cam"], j["W"], j["H"], j["params"], base_seed=j["seed"])
+            t1 = time.perf_counter()
+            b.render(j["spp"])
+            b.read()
+            t_batch += time.perf_counter() - t1
+            b.release()
+    lat = np.array(lat)
+    served = len(jobs) / makespan
+    plain = n_plain / t_frame
+    plain_b = n_plain / t_batch
+    line = {
+        "metric": "served jobs/s with p50/p99 latency vs per-job" if serve else "batched jobs/s vs per-job",
+        "value": round(served, 3), "unit": "jobs/s", "n_gpus": 1, "steps": len(jobs), "warmup": 1,
+        "ms_per_step": round(makespan * 1e3 / len(jobs), 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": ("Poisson arrivals at %.1f jobs/s for %.1f s" % (args.rate, args.duration) if serve else
+                         "%d jobs submitted together" % len(jobs)) +
+                        ", 8 icosphere scenes (20k-2M tris), random cameras, 128^2-1024^2, 16-256 spp, depth 8, "
+                        "NEE+MIS, 64 GB image budget",
+            "jobs": len(jobs), "scene_tris": [len(s.tris) for s in scenes],
+            "total_spp_megasamples": round(sum(j["spp"] * j["W"] * j["H"] for j in jobs) / 1e6, 1),
+        },
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1), "p99": round(float(np.percentile(lat, 99)), 1),
+                       "max": round(float(lat.max()), 1)},
+        "makespan_s": round(makespan, 3),
+        "served_mrays_per_s": round((sst.rays_ext + sst.rays_shadow) / makespan / 1e6, 1),
+        "plain": {"jobs_per_s_frame_at_a_time": round(plain, 3), "jobs_per_s_job_at_a_time": round(plain_b, 3),
+                  "sample_jobs": n_plain, "speedup_vs_frame_at_a_time": round(served / plain, 2),
+                  "speedup_vs_job_at_a_time": round(served / plain_b, 2),
+                  "images_identical_to_served": same},
+    }
+    print(json.dumps(line))
 
 
 def make_scene(Y, name):
@@ -116,7 +246,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the workload's default)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="icosphere")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch", "serve"], default="icosphere")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrival rate (jobs/s)")
+    ap.add_argument("--duration", type=float, default=3.0, help="serve: seconds of arrivals")
+    ap.add_argument("--jobs", type=int, default=0, help="batch / serve: number of jobs (0: 64 / rate x duration)")
+    ap.add_argument("--plain-jobs", type=int, default=16, help="jobs timed on the plain per-job paths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--trace-block", type=int, default=0)
@@ -138,6 +272,9 @@ def main():
 
     import yune_amd as Y
 
+    if args.workload in ("batch", "serve"):
+        serve_bench(args, Y, np)
+        return
     wl = WORKLOADS[args.workload]
     W, H, F = wl["W"], wl["H"], wl["frames"]
     steps = args.steps or wl["steps"]

@@ -13,7 +13,11 @@ Tolerances:
     (a tie, resolved by test order), which are counted and must be rare;
   * shadow rays: same occluded flag, and the GPU's occluder must be a real
     occluder according to the oracle;
-  * radiance: mean over pixels of |gpu - cpu|_1 / max(|cpu|_1, 1e-2) <= 1%.
+  * radiance: bit-identical in at least 99.9% of pixels, and mean over
+    pixels of |gpu - cpu|_1 / max(|cpu|_1, 1e-2) <= 1%.  The kernels and
+    the oracle share one definition of sin, cos and pow (yn_fmath.h) and
+    every other float operation rounds once in the reference's order, so
+    only closest-hit ties (equal t on two triangles) can make paths differ.
 """
 import numpy as np
 import pytest
@@ -138,6 +142,10 @@ def test_primary_and_shadow_rays_match(mis):
     assert hit.mean() > 0.5 and rec.mean() > 0.4
 
 
+def identical(a, b):
+    return float((a.view(np.int32) == b.view(np.int32)).all(-1).mean())
+
+
 def mre(a, b):
     a = a[..., :3].astype(np.float64)
     b = b[..., :3].astype(np.float64)
@@ -158,6 +166,7 @@ def test_radiance_cornell(mis, fix):
         gimg = job.read()
     oimg, _, _, _ = oscene(s, s.bvh()).render(ocam(s.cam), W, H, O.params(0, **kw), O.mt19937(7, F), threads=8)
     assert np.array_equal(gimg[..., 3], oimg[..., 3])        # sample counts
+    assert identical(gimg, oimg) >= 0.999
     e, nonfinite = mre(gimg, oimg)
     assert e <= 0.01, e
     assert nonfinite <= 0.01
@@ -173,6 +182,7 @@ def test_radiance_icosphere_room():
         job.render(F)
         gimg = job.read()
     oimg, _, _, _ = oscene(s, s.bvh()).render(ocam(s.cam), W, H, O.params(0, **kw), O.mt19937(11, F), threads=8)
+    assert identical(gimg, oimg) >= 0.999
     e, nonfinite = mre(gimg, oimg)
     assert e <= 0.01, e
 
@@ -186,6 +196,7 @@ def test_reference_defaults_unbounded_loop():
         job.render(F)
         gimg = job.read()
     oimg, _, _, _ = oscene(s, s.bvh()).render(ocam(s.cam), W, H, O.params(0), O.mt19937(5, F), threads=8)
+    assert identical(gimg, oimg) >= 0.999
     e, _ = mre(gimg, oimg)
     assert e <= 0.01, e
 

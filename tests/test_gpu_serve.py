@@ -108,3 +108,61 @@ def test_admission_waits_for_memory():
             alone.render(3)
             assert np.array_equal(done[k].view(np.int32), alone.read().view(np.int32))
             alone.release()
+
+
+def _read_png(path):
+    import struct
+    import zlib
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, w = 8, b"", h = None
+    w = h = None
+    while pos < len(data):
+        n, = struct.unpack(">I", data[pos:pos + 4])
+        typ, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        crc, = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])
+        assert zlib.crc32(typ + body) == crc
+        if typ == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", body[:10])
+            assert (depth, ctype) == (8, 2)
+        elif typ == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = zlib.decompress(idat)
+    rows = np.frombuffer(raw, np.uint8).reshape(h, 1 + 3 * w)
+    assert (rows[:, 0] == 0).all()                 # filter type 0
+    return rows[:, 1:].reshape(h, w, 3)
+
+
+def _read_hdr(path):
+    data = open(path, "rb").read()
+    head, _, rest = data.partition(b"\n\n")
+    assert head.startswith(b"#?RADIANCE") and b"FORMAT=32-bit_rle_rgbe" in head
+    res, _, px = rest.partition(b"\n")
+    _, h, _, w = res.split()
+    h, w = int(h), int(w)
+    e = np.frombuffer(px, np.uint8).reshape(h, w, 4).astype(np.float64)
+    scale = np.where(e[..., 3:] > 0, np.ldexp(1.0, (e[..., 3:] - 136).astype(int)), 0.0)
+    return (e[..., :3] + 0.5) * scale * (e[..., 3:] > 0)
+
+
+def test_save_hdr_png_flip_and_tonemap(tmp_path):
+    """RendererCore::saveImage: the file's first row is the image's last
+    (vertical flip), .hdr holds the linear image, .png the tonemapped one
+    as round(clamp(v) * 255); the device tonemap equals the oracle's."""
+    s = Y.gen_cornell()
+    W, H = 40, 24
+    with Y.Context(max_paths=1 << 14) as c:
+        job = c.job(c.scene(s), s.cam, W, H, Y.params_reference(mis=1, max_bounces=8, rr_threshold=2), base_seed=3)
+        job.render(4)
+        img = job.read()
+        tm = job.read_tonemapped()
+        job.save(str(tmp_path / "a.hdr"))
+        job.save(str(tmp_path / "a.png"))
+    assert np.array_equal(tm.view(np.int32), O.tonemap(img).view(np.int32))
+    png = _read_png(tmp_path / "a.png")
+    exp = np.rint(np.clip(np.nan_to_num(tm[..., :3], nan=0.0), 0, 1) * 255).astype(np.uint8)[::-1]
+    assert np.abs(png.astype(int) - exp.astype(int)).max() <= 1    # lround vs rint on exact halves
+    hdr = _read_hdr(tmp_path / "a.hdr")
+    ref = img[..., :3][::-1].astype(np.float64)
+    assert np.allclose(hdr, ref, rtol=1 / 128, atol=1e-6)

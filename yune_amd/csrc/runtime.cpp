@@ -606,19 +606,40 @@ int serve_round(yn_ctx* c, int* frames_out)
             return 0;
         }
     }
+    // Processor sharing: every waiting job of the round's mode first gets an
+    // equal share of the path capacity (at least one frame, oldest first),
+    // then what is left goes to the oldest jobs.  Small jobs finish within a
+    // round or two; large ones keep the batch full.
+    std::vector<int> cand;
+    for (int jid : order) if (c->jobs[jid].p.mode == mode) cand.push_back(jid);
+    std::vector<int64_t> kf(cand.size(), 0);
+    int64_t used = 0;
+    const int64_t share = pcap / (int64_t)cand.size();
+    for (size_t i = 0; i < cand.size(); ++i) {
+        const JobRec& J = c->jobs[cand[i]];
+        int64_t px = (int64_t)J.W * J.H;
+        int64_t want = std::min<int64_t>(J.target - (int64_t)J.frame, std::max<int64_t>(1, share / px));
+        int64_t k = std::min<int64_t>(want, (pcap - used) / px);
+        if (k > 0) { kf[i] = k; used += k * px; }
+    }
+    for (size_t i = 0; i < cand.size() && used < pcap; ++i) {
+        const JobRec& J = c->jobs[cand[i]];
+        int64_t px = (int64_t)J.W * J.H;
+        int64_t k = std::min<int64_t>(J.target - (int64_t)J.frame - kf[i], (pcap - used) / px);
+        if (k > 0) { kf[i] += k; used += k * px; }
+    }
     struct Plan { int jid; int k; int64_t off; bool reset; };
     std::vector<Plan> plan;
     std::vector<Region> regs;
     int64_t off = 0;
-    for (int jid : order) {
+    for (size_t i = 0; i < cand.size(); ++i) {
+        if (kf[i] == 0) continue;
+        int jid = cand[i];
         JobRec& J = c->jobs[jid];
-        if (J.p.mode != mode) continue;
         int64_t px = (int64_t)J.W * J.H;
-        int64_t k = std::min<int64_t>(J.target - (int64_t)J.frame, (pcap - off) / px);
-        if (k <= 0) continue;
-        for (int64_t f = 0; f < k; ++f) regs.push_back({jid, (int32_t)f, (uint32_t)J.engine(), 0, 0, J.W, J.H});
-        plan.push_back({jid, (int)k, off, J.frame == 0});
-        off += k * px;
+        for (int64_t f = 0; f < kf[i]; ++f) regs.push_back({jid, (int32_t)f, (uint32_t)J.engine(), 0, 0, J.W, J.H});
+        plan.push_back({jid, (int)kf[i], off, J.frame == 0});
+        off += kf[i] * px;
         if (J.t_start < 0) J.t_start = ms_since(c);
     }
     if (regs.empty()) return 0;
