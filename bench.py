@@ -135,7 +135,7 @@ def serve_bench(args, Y, np):
         makespan = max(done_t) / 1e3
         sst = ctx.serve_stats()
         # plain paths on a sample of the same jobs
-        n_plain = min(len(jobs), args.plain_jobs)
+        n_plain = min(len(jobs), args.plain_jobs) if args.plain_jobs > 0 else len(jobs)
         t_frame = t_batch = 0.0
         same = True
         for k in range(n_plain):
@@ -250,7 +250,7 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrival rate (jobs/s)")
     ap.add_argument("--duration", type=float, default=3.0, help="serve: seconds of arrivals")
     ap.add_argument("--jobs", type=int, default=0, help="batch / serve: number of jobs (0: 64 / rate x duration)")
-    ap.add_argument("--plain-jobs", type=int, default=16, help="jobs timed on the plain per-job paths")
+    ap.add_argument("--plain-jobs", type=int, default=0, help="jobs timed on the plain per-job paths (0: all)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--trace-block", type=int, default=0)
@@ -258,6 +258,8 @@ def main():
     ap.add_argument("--trace-grid", type=int, default=0)
     ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--shade-waves", type=int, default=0)
+    ap.add_argument("--max-paths", type=int, default=0, help="paths in flight per batch (0: library default)")
+    ap.add_argument("--frames", type=int, default=0, help="frames per step (0: the workload's default)")
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC counters")
     args = ap.parse_args()
     if args.copy_probe:
@@ -276,7 +278,7 @@ def main():
         serve_bench(args, Y, np)
         return
     wl = WORKLOADS[args.workload]
-    W, H, F = wl["W"], wl["H"], wl["frames"]
+    W, H, F = wl["W"], wl["H"], args.frames or wl["frames"]
     steps = args.steps or wl["steps"]
     t0 = time.perf_counter()
     scene = make_scene(Y, args.workload)
@@ -287,7 +289,8 @@ def main():
 
     params = Y.params_reference(wl["mode"], **wl["params"])
     with Y.Context(bvh_width=args.bvh_width, trace_block=args.trace_block, trace_chunk=args.trace_chunk,
-                   trace_grid=args.trace_grid, lds_stack=args.lds_stack, shade_waves=args.shade_waves) as ctx:
+                   trace_grid=args.trace_grid, lds_stack=args.lds_stack, shade_waves=args.shade_waves,
+                   max_paths=args.max_paths) as ctx:
         sid = ctx.scene(scene, nodes)
         info = ctx.scene_info(sid)
         # counting pass: exact node visits / triangle tests of one step

@@ -181,6 +181,22 @@ int  yn_job_status_get(yn_ctx* ctx, int32_t job_id, yn_job_status* out);
 int  yn_serve_stats(yn_ctx* ctx, yn_stats* out);     /* rays and kernel times of all rounds */
 double yn_clock_ms(yn_ctx* ctx);                     /* ms since yn_create */
 
+/* ---- autotuning: no counterpart in the reference (its launch shape is
+ * fixed: one work-item per pixel of a tile).  yn_autotune renders short
+ * timed runs of the given view (frames per run) and keeps, knob by knob,
+ * the fastest BVH width (the scenes' GPU layout is rebuilt), trace block
+ * size, rays per queue fetch, LDS stack depth, shade occupancy and paths
+ * per batch in the context's configuration.  Not while served jobs are
+ * pending.  yn_config_get reads the configuration in effect. ---- */
+typedef struct yn_tune_result {
+    int32_t bvh_width, trace_block, trace_chunk, lds_stack, shade_waves, n_tried;
+    int64_t batch_paths;
+    double  mrays_per_s;      /* the chosen configuration on the tuning view */
+} yn_tune_result;
+int  yn_autotune(yn_ctx* ctx, int32_t scene_id, const yn_cam* cam, int32_t width, int32_t height,
+                 const yn_render_params* params, int32_t frames, yn_tune_result* out);
+int  yn_config_get(yn_ctx* ctx, yn_config* out, int64_t* batch_paths);
+
 /* ---- diagnostics: the kernels' sin (op 0), cos (op 1) and pow (op 2) on
  * the device for n host inputs (x, and y for pow).  OpenCL leaves these
  * built-ins' exact results to the runtime; this library defines each once

@@ -167,3 +167,29 @@ def test_save_hdr_png_flip_and_tonemap(tmp_path):
     ref = img[..., :3][::-1].astype(np.float64)
     # RGBE: one shared exponent per pixel, 8-bit mantissas
     assert (np.abs(hdr - ref) <= ref.max(-1, keepdims=True) / 128 + 1e-6).all()
+
+
+def test_autotune_chooses_valid_config_and_keeps_results():
+    """yn_autotune tries every knob (BVH width rebuilds the scene layout) and
+    leaves a configuration under which images stay bit-identical to the
+    oracle."""
+    s = Y.gen_icosphere_room(3, 9, 2)
+    W, H = 80, 45
+    kw = dict(mis=1, max_bounces=8, rr_threshold=2)
+    p = Y.params_reference(**kw)
+    with Y.Context(max_paths=1 << 16) as c:
+        sid = c.scene(s)
+        r = c.autotune(sid, s.cam, W, H, p, frames=2)
+        assert r["n_tried"] >= 12 and r["mrays_per_s"] > 0
+        assert r["bvh_width"] in (2, 4) and r["trace_block"] in (64, 128, 256)
+        assert r["lds_stack"] in (8, 16, 32) and r["shade_waves"] in (2, 3, 4)
+        cfg = c.config()
+        assert (cfg["bvh_width"], cfg["trace_chunk"], cfg["batch_paths"]) == (r["bvh_width"], r["trace_chunk"],
+                                                                               r["batch_paths"])
+        assert c.scene_info(sid).width == r["bvh_width"]
+        job = c.job(sid, s.cam, W, H, p, base_seed=11)
+        job.render(4)
+        img = job.read()
+    oimg, _, _, _ = O.OScene(s.tris, s.mats, s.bvh(), s.light, heap_size=0).render(
+        O.Cam.from_buffer_copy(bytes(s.cam)), W, H, O.params(0, **kw), O.mt19937(11, 4), threads=8)
+    assert (img.view(np.int32) == oimg.view(np.int32)).all(-1).mean() >= 0.999

@@ -22,6 +22,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._lib import (MAT_DTYPE, NODE_DTYPE, TRI_DTYPE, YN_MODE_BDPT, YN_MODE_PT, Cam, Config, HostScene, JobStatus,
+                   TuneResult,
                    LightQuad, RenderParams, SceneInfo, Stats, YuneError, check, lib, ptr)
 
 __all__ = ["TRI_DTYPE", "MAT_DTYPE", "NODE_DTYPE", "YN_MODE_PT", "YN_MODE_BDPT", "YuneError", "Scene",
@@ -217,6 +218,21 @@ class Context:
         st = Stats()
         self._ck(lib().yn_serve_stats(self._p, C.byref(st)))
         return st
+
+    def autotune(self, sid: int, cam: Cam, width: int, height: int, params: RenderParams, frames: int = 2) -> dict:
+        """Choose BVH width, launch shape, LDS stack depth, shade occupancy and
+        batch size for this view (yn_autotune); the context keeps them."""
+        r = TuneResult()
+        self._ck(lib().yn_autotune(self._p, sid, C.byref(cam), width, height, C.byref(params), frames, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TuneResult._fields_}
+
+    def config(self) -> dict:
+        cfg = Config()
+        bp = C.c_int64()
+        self._ck(lib().yn_config_get(self._p, C.byref(cfg), C.byref(bp)))
+        d = {k: getattr(cfg, k) for k, _ in Config._fields_}
+        d["batch_paths"] = bp.value
+        return d
 
     def clock_ms(self) -> float:
         return lib().yn_clock_ms(self._p)

@@ -67,6 +67,12 @@ class SceneInfo(C.Structure):
                 ("tri_bytes", C.c_size_t), ("shade_bytes", C.c_size_t)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("bvh_width", C.c_int32), ("trace_block", C.c_int32), ("trace_chunk", C.c_int32),
+                ("lds_stack", C.c_int32), ("shade_waves", C.c_int32), ("n_tried", C.c_int32),
+                ("batch_paths", C.c_int64), ("mrays_per_s", C.c_double)]
+
+
 class JobStatus(C.Structure):
     _fields_ = [("frames_target", C.c_int32), ("frames_done", C.c_int32), ("admitted", C.c_int32),
                 ("done", C.c_int32), ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double),
@@ -126,6 +132,8 @@ SIGNATURES = {
     "yn_serve_stats": (I32, [P, C.POINTER(Stats)]),
     "yn_clock_ms": (C.c_double, [P]),
     "yn_fmath_eval": (I32, [P, I32, P, P, I32, P]),
+    "yn_autotune": (I32, [P, I32, C.POINTER(Cam), I32, I32, C.POINTER(RenderParams), I32, C.POINTER(TuneResult)]),
+    "yn_config_get": (I32, [P, C.POINTER(Config), C.POINTER(C.c_int64)]),
 }
 
 _lib = None

@@ -39,6 +39,11 @@ struct SceneRec {
     DevScene dev{};
     void* bufs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     yn_scene_info info{};
+    // the caller's arrays, kept so the GPU layout can be rebuilt (BVH width)
+    std::vector<yn_triangle> h_tris;
+    std::vector<yn_material> h_mats;
+    std::vector<yn_bvh_node> h_nodes;
+    yn_light_quad h_light{};
 };
 
 struct JobRec {
@@ -115,6 +120,7 @@ struct yn_ctx {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     int64_t submit_seq = 0;
     JobRec serve_rec;                 // statistics of scheduler rounds
+    int64_t batch_paths = 0;          // paths per wavefront batch (0: the whole capacity)
 };
 
 namespace {
@@ -493,6 +499,7 @@ int render_rect(yn_ctx* c, int jid, float4* accum, int x0, int y0, int w, int h,
         if (int r = ensure_bdpt(c, J.p.bdpt_bounces)) return r;
         pcap = c->cap_bdpt;
     }
+    if (c->batch_paths > 0) pcap = std::min(pcap, c->batch_paths);
     // Sample slots are indexed [slot][W*H]; a batch may hold K frames when
     // K full images fit in the path capacity.
     int64_t kcap = pcap / std::max<int64_t>(full, 1);
@@ -589,6 +596,7 @@ int serve_round(yn_ctx* c, int* frames_out)
         if (int r = ensure_bdpt(c, max_kb)) return r;
         pcap = c->cap_bdpt;
     }
+    if (c->batch_paths > 0) pcap = std::min(pcap, c->batch_paths);
     // a job whose frame exceeds the batch runs alone, in row bands
     {
         JobRec& J = c->jobs[order[0]];
@@ -662,6 +670,69 @@ int serve_round(yn_ctx* c, int* frames_out)
         if ((int64_t)J.frame >= J.target) J.t_done = now;
         *frames_out += pl.k;
     }
+    return 0;
+}
+
+// GPU layout of scene `sid` at `width` from its kept host arrays, uploaded
+// in place of the previous one (the scene id stays valid).
+int build_scene(yn_ctx* c, int sid, int width)
+{
+    SceneRec& S = c->scenes[sid];
+    GpuLayout L;
+    std::string err;
+    LayoutOpts lo;
+    lo.width = width;
+    lo.hoist_frac = c->cfg.hoist_frac;
+    lo.leaf_max = c->cfg.leaf_max;
+    int r = make_layout(S.h_tris.data(), (int32_t)S.h_tris.size(), (int32_t)S.h_mats.size(), S.h_nodes.data(),
+                        (int32_t)S.h_nodes.size(), lo, L, err);
+    if (r) return fail(c, r, err);
+    if (L.max_depth > kMaxStack) return fail(c, YN_ERR_INVALID, "BVH deeper than the traversal stack bound");
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    void* nb[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    auto up = [&](int k, const void* src, size_t bytes) -> int {
+        HIPCK(c, hipMalloc(&nb[k], std::max<size_t>(bytes, 16)));
+        if (bytes) HIPCK(c, hipMemcpy(nb[k], src, bytes, hipMemcpyHostToDevice));
+        return 0;
+    };
+    if ((r = up(0, L.nodes.data(), L.nodes.size() * 4)) || (r = up(1, L.tri_isect.data(), L.tri_isect.size() * 4)) ||
+        (r = up(2, L.tri_shade.data(), L.tri_shade.size() * 4)) || (r = up(3, L.tri_mat.data(), L.tri_mat.size() * 4)) ||
+        (r = up(4, S.h_mats.data(), sizeof(yn_material) * S.h_mats.size())) ||
+        (r = up(5, L.hoist_box.data(), L.hoist_box.size() * 4))) {
+        for (void*& b : nb) { if (b) hipFree(b); b = nullptr; }
+        return r;
+    }
+    for (int k = 0; k < 6; ++k) {
+        if (S.bufs[k]) hipFree(S.bufs[k]);
+        S.bufs[k] = nb[k];
+    }
+    DevScene& D = S.dev;
+    D.nodes = (const float4*)S.bufs[0];
+    D.tri_isect = (const float4*)S.bufs[1];
+    D.tri_shade = (const float4*)S.bufs[2];
+    D.tri_mat = (const int32_t*)S.bufs[3];
+    D.mats = (const yn_material*)S.bufs[4];
+    D.hoist_box = (const float4*)S.bufs[5];
+    D.light = S.h_light;
+    // the reference constant has these w's (udpt.cl:97-104)
+    D.light.pos.w = 1.0f; D.light.normal.w = 0.0f; D.light.ke.w = 0.0f;
+    D.light.edge_l.w = 0.0f; D.light.edge_w.w = 0.0f;
+    D.root_ref = L.root_ref;
+    std::memcpy(D.root_box, L.root_box, sizeof D.root_box);
+    D.width = L.width;
+    D.brute = L.brute;
+    D.n_slots = (int32_t)L.slot_tri.size();
+    D.n_hoist = L.n_hoist;
+    HIPCK(c, hipMemcpy(c->d_scenes + sid, &D, sizeof(DevScene), hipMemcpyHostToDevice));
+    S.info.n_tri = (int32_t)S.h_tris.size();
+    S.info.n_ref_nodes = (int32_t)S.h_nodes.size();
+    S.info.n_gpu_nodes = L.n_nodes;
+    S.info.width = L.width;
+    S.info.max_depth = L.max_depth;
+    S.info.n_hoist = L.n_hoist;
+    S.info.node_bytes = L.nodes.size() * 4;
+    S.info.tri_bytes = L.tri_isect.size() * 4;
+    S.info.shade_bytes = L.tri_shade.size() * 4 + L.tri_mat.size() * 4;
     return 0;
 }
 
@@ -787,62 +858,19 @@ int yn_scene_create(yn_ctx* c, const yn_triangle* tris, int32_t n_tri, const yn_
                     const yn_bvh_node* nodes, int32_t n_node, const yn_light_quad* light, int32_t* scene_id)
 {
     if (check_ctx(c)) return YN_ERR_INVALID;
-    if (!light || !scene_id || n_mat <= 0 || !mats) return fail(c, YN_ERR_INVALID, "bad scene arguments");
+    if (!light || !scene_id || n_mat <= 0 || !mats || n_tri < 0 || n_node < 0 || (n_tri > 0 && !tris) ||
+        (n_node > 0 && !nodes))
+        return fail(c, YN_ERR_INVALID, "bad scene arguments");
     int sid = -1;
     for (int i = 0; i < kMaxScenes; ++i) if (!c->scenes[i].live) { sid = i; break; }
     if (sid < 0) return fail(c, YN_ERR_NO_MEMORY, "too many scenes");
-    GpuLayout L;
-    std::string err;
-    LayoutOpts lo;
-    lo.width = c->cfg.bvh_width;
-    lo.hoist_frac = c->cfg.hoist_frac;
-    lo.leaf_max = c->cfg.leaf_max;
-    int r = make_layout(tris, n_tri, n_mat, nodes, n_node, lo, L, err);
-    if (r) return fail(c, r, err);
-    if (L.max_depth > kMaxStack) return fail(c, YN_ERR_INVALID, "BVH deeper than the traversal stack bound");
     SceneRec& S = c->scenes[sid];
     S = SceneRec();
-    auto up = [&](int k, const void* src, size_t bytes) -> int {
-        void* d = nullptr;
-        HIPCK(c, hipMalloc(&d, std::max<size_t>(bytes, 16)));
-        S.bufs[k] = d;
-        if (bytes) HIPCK(c, hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
-        return 0;
-    };
-    if ((r = up(0, L.nodes.data(), L.nodes.size() * 4)) || (r = up(1, L.tri_isect.data(), L.tri_isect.size() * 4)) ||
-        (r = up(2, L.tri_shade.data(), L.tri_shade.size() * 4)) || (r = up(3, L.tri_mat.data(), L.tri_mat.size() * 4)) ||
-        (r = up(4, mats, sizeof(yn_material) * (size_t)n_mat)) ||
-        (r = up(5, L.hoist_box.data(), L.hoist_box.size() * 4))) {
-        for (void*& b : S.bufs) { if (b) hipFree(b); b = nullptr; }
-        return r;
-    }
-    DevScene& D = S.dev;
-    D.nodes = (const float4*)S.bufs[0];
-    D.tri_isect = (const float4*)S.bufs[1];
-    D.tri_shade = (const float4*)S.bufs[2];
-    D.tri_mat = (const int32_t*)S.bufs[3];
-    D.mats = (const yn_material*)S.bufs[4];
-    D.hoist_box = (const float4*)S.bufs[5];
-    D.light = *light;
-    // the reference constant has these w's (udpt.cl:97-104)
-    D.light.pos.w = 1.0f; D.light.normal.w = 0.0f; D.light.ke.w = 0.0f;
-    D.light.edge_l.w = 0.0f; D.light.edge_w.w = 0.0f;
-    D.root_ref = L.root_ref;
-    std::memcpy(D.root_box, L.root_box, sizeof D.root_box);
-    D.width = L.width;
-    D.brute = L.brute;
-    D.n_slots = (int32_t)L.slot_tri.size();
-    D.n_hoist = L.n_hoist;
-    HIPCK(c, hipMemcpy(c->d_scenes + sid, &D, sizeof(DevScene), hipMemcpyHostToDevice));
-    S.info.n_tri = n_tri;
-    S.info.n_ref_nodes = n_node;
-    S.info.n_gpu_nodes = L.n_nodes;
-    S.info.width = L.width;
-    S.info.max_depth = L.max_depth;
-    S.info.n_hoist = L.n_hoist;
-    S.info.node_bytes = L.nodes.size() * 4;
-    S.info.tri_bytes = L.tri_isect.size() * 4;
-    S.info.shade_bytes = L.tri_shade.size() * 4 + L.tri_mat.size() * 4;
+    S.h_tris.assign(tris, tris + n_tri);
+    S.h_mats.assign(mats, mats + n_mat);
+    S.h_nodes.assign(nodes, nodes + n_node);
+    S.h_light = *light;
+    if (int r = build_scene(c, sid, c->cfg.bvh_width)) { S = SceneRec(); return r; }
     S.live = true;
     *scene_id = sid;
     return 0;
@@ -855,7 +883,7 @@ int yn_scene_release(yn_ctx* c, int32_t sid)
         if (j.live && j.scene == sid) return fail(c, YN_ERR_INVALID, "scene still used by a job");
     HIPCK(c, hipStreamSynchronize(c->stream));
     for (void*& b : c->scenes[sid].bufs) { if (b) hipFree(b); b = nullptr; }
-    c->scenes[sid].live = false;
+    c->scenes[sid] = SceneRec();
     return 0;
 }
 
@@ -1122,6 +1150,113 @@ int yn_serve_stats(yn_ctx* c, yn_stats* out)
 }
 
 double yn_clock_ms(yn_ctx* c) { return c ? ms_since(c) : 0.0; }
+
+int yn_config_get(yn_ctx* c, yn_config* out, int64_t* batch_paths)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (!out) return fail(c, YN_ERR_INVALID, "null output");
+    *out = c->cfg;
+    if (batch_paths) *batch_paths = c->batch_paths > 0 ? c->batch_paths : c->cap_paths;
+    return 0;
+}
+
+// Coordinate descent over the launch and layout knobs on short timed
+// renders of the caller's view: each knob in turn takes each of its values
+// with the others held, and keeps a value that is more than 1% faster.
+int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H, const yn_render_params* p,
+                int32_t frames, yn_tune_result* out)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (check_scene(c, sid)) return YN_ERR_NOT_FOUND;
+    if (!out || frames <= 0 || !valid_params(p) || !valid_cam(cam) || W <= 0 || H <= 0)
+        return fail(c, YN_ERR_INVALID, "bad autotune arguments");
+    for (const JobRec& J : c->jobs)
+        if (J.live && J.served && (int64_t)J.frame < J.target)
+            return fail(c, YN_ERR_INVALID, "autotune while served jobs are pending");
+    int tried = 0;
+    int err = 0;
+    auto measure = [&]() -> double {
+        int32_t jid = -1;
+        if ((err = yn_job_create(c, sid, cam, W, H, p, 0x9e3779b9u, &jid))) return -1.0;
+        double rate = -1.0;
+        if (!(err = yn_job_render(c, jid, 1))) {
+            yn_stats a = c->jobs[jid].st;
+            auto t0 = std::chrono::steady_clock::now();
+            if (!(err = yn_job_render(c, jid, frames))) {
+                double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                yn_stats b = c->jobs[jid].st;
+                rate = (double)((b.rays_ext - a.rays_ext) + (b.rays_shadow - a.rays_shadow)) / std::max(dt, 1e-9);
+            }
+        }
+        yn_job_release(c, jid);
+        ++tried;
+        return rate;
+    };
+    auto set_width = [&](int w) -> int {
+        for (int i = 0; i < kMaxScenes; ++i)
+            if (c->scenes[i].live)
+                if (int r = build_scene(c, i, w)) return r;
+        c->cfg.bvh_width = w;
+        return 0;
+    };
+    const int64_t cap = c->cap_paths;
+    struct Knob { int kind; std::vector<int64_t> vals; };
+    std::vector<Knob> knobs = {
+        {0, {4, 2}},                            // BVH width
+        {1, {256, 128, 64}},                    // trace block
+        {2, {64, 32, 128}},                     // rays per wave per queue fetch
+        {3, {16, 8, 32}},                       // LDS stack entries
+        {4, {2, 3, 4}},                         // shade occupancy
+        {5, {cap, cap / 2, cap / 4}},           // paths per batch
+    };
+    auto get = [&](int kind) -> int64_t {
+        switch (kind) {
+        case 0: return c->cfg.bvh_width;
+        case 1: return c->cfg.trace_block;
+        case 2: return c->cfg.trace_chunk;
+        case 3: return c->cfg.lds_stack;
+        case 4: return c->cfg.shade_waves;
+        default: return c->batch_paths > 0 ? c->batch_paths : cap;
+        }
+    };
+    auto set = [&](int kind, int64_t v) -> int {
+        switch (kind) {
+        case 0: return set_width((int)v);
+        case 1: c->cfg.trace_block = (int)v; return 0;
+        case 2: c->cfg.trace_chunk = (int)v; return 0;
+        case 3: c->cfg.lds_stack = (int)v; return 0;
+        case 4: c->cfg.shade_waves = (int)v; return 0;
+        default: c->batch_paths = v; return 0;
+        }
+    };
+    double best = measure();
+    if (best < 0) return err ? err : fail(c, YN_ERR_HIP, "autotune: first render failed");
+    for (int pass = 0; pass < 2; ++pass) {
+        bool changed = false;
+        for (const Knob& k : knobs) {
+            int64_t keep = get(k.kind);
+            for (int64_t v : k.vals) {
+                if (v == keep || v <= 0) continue;
+                if (int r = set(k.kind, v)) return r;
+                double m = measure();
+                if (m < 0) return err;
+                if (m > best * 1.01) { best = m; keep = v; changed = true; }
+            }
+            if (get(k.kind) != keep)
+                if (int r = set(k.kind, keep)) return r;
+        }
+        if (!changed) break;
+    }
+    out->bvh_width = c->cfg.bvh_width;
+    out->trace_block = c->cfg.trace_block;
+    out->trace_chunk = c->cfg.trace_chunk;
+    out->lds_stack = c->cfg.lds_stack;
+    out->shade_waves = c->cfg.shade_waves;
+    out->batch_paths = c->batch_paths > 0 ? c->batch_paths : cap;
+    out->mrays_per_s = best / 1e6;
+    out->n_tried = tried;
+    return 0;
+}
 
 int yn_fmath_eval(yn_ctx* c, int32_t op, const float* x, const float* y, int32_t n, float* out)
 {
