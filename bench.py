@@ -3,9 +3,9 @@
 Default workload (the headline): the ~1M-triangle 1920x1080 scene (48
 icospheres of 20,480 triangles in a 10x10x10 room, 2-triangle quad light),
 path tracing with NEE+MIS, max depth 8, Russian roulette from bounce 3.
-A step is one batch of the wavefront path tracer: 4 frames of 1920x1080 at
-1 sample per pixel each (8.3M paths) run from primary rays until every path
-has ended, then folded into the progressive image; 16 steps are the 64 spp
+A step is one batch of the wavefront path tracer: 8 frames of 1920x1080 at
+1 sample per pixel each (16.6M paths) run from primary rays until every path
+has ended, then folded into the progressive image; 8 steps are the 64 spp
 of the workload.
 
 Other single-GPU workloads (--workload):
@@ -41,7 +41,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_ext_1m_1080p_traffic.txt"
 
 WORKLOADS = {
     "icosphere": dict(
-        metric="Mrays/s at 1M tris 1080p", W=1920, H=1080, frames=4, steps=16, mode=0,
+        metric="Mrays/s at 1M tris 1080p", W=1920, H=1080, frames=8, steps=8, mode=0,
         params=dict(mis=1, max_bounces=8, rr_threshold=2),
         desc="path tracing, 48 icospheres x 20480 tris + 12-tri room + 2-tri quad light ({n} tris), 1920x1080, "
              "{f} spp per step, max depth 8, RR from bounce 3, NEE+MIS"),
