@@ -265,3 +265,21 @@ def test_deep_tree_uses_stack_overflow_area(width):
     otri, olight, ot, _ = oscene(s, s.nodes).trace(rays)
     assert np.array_equal(gtri, otri) and np.array_equal(gt.view(np.int32), ot.view(np.int32))
     assert (gtri == 0).all()
+
+
+def test_regeneration_with_work_beyond_capacity():
+    """16 frames of 64x64 through 4096 paths in flight: ended paths' slots
+    are regenerated from the remaining work items, and the image is still
+    the oracle's bit for bit."""
+    s = Y.gen_cornell()
+    W, H, F = 64, 64, 16
+    kw = dict(mis=1, max_bounces=8, rr_threshold=2)
+    with Y.Context(max_paths=1 << 12) as c:
+        job = _cornell_job(c, s, W, H, Y.params_reference(**kw), 21)
+        job.render(F)
+        gimg = job.read()
+        st = job.stats()
+    oimg, _, _, _ = oscene(s, s.bvh()).render(ocam(s.cam), W, H, O.params(0, **kw), O.mt19937(21, F), threads=8)
+    assert st.samples == W * H * F
+    assert np.array_equal(gimg[..., 3], oimg[..., 3])
+    assert identical(gimg, oimg) >= 0.999

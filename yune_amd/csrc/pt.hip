@@ -52,6 +52,7 @@ constexpr float kPinkR = 0.988f, kPinkG = 0.0588f, kPinkB = 0.7529f;   // udpt.c
 
 struct Out {
     int kind = 0;           // 0 none, 1 extension, 2 shadow
+    bool done = false;      // the path ended and wrote its sample
     float4 o_t;
     float4 d_p;
 };
@@ -332,6 +333,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
         write_sample(J, slot, pix, mk(NAN, NAN, NAN, NAN));
         st = ST_DONE;
     }
+    out.done = out.kind == 0;
 
     P.state[p] = st;
     P.seed[p] = seed;
@@ -366,32 +368,49 @@ __global__ __launch_bounds__(256, WAVES) void k_shade_pt(ShadeArgs a, DevCapture
     for (uint32_t base = wave * 64; base < total; base += n_waves * 64) {
         uint32_t i = base + (uint32_t)lane;
         Out o;
+        int pid = -1;
         if (i < total) {
             bool ext = i < n_ext;
             uint32_t k = ext ? i : i - n_ext;
             float4 ro = ext ? a.ext_in.o_t[k] : a.sh_in.o_t[k];
             float4 rdp = ext ? a.ext_in.d_p[k] : a.sh_in.d_p[k];
-            o = advance(a, cap, __float_as_int(rdp.w), ext, ro, rdp);
+            pid = __float_as_int(rdp.w);
+            o = advance(a, cap, pid, ext, ro, rdp);
         }
         uint32_t e = wave_append(a.ext_out.count, o.kind == 1);
         if (o.kind == 1) { a.ext_out.o_t[e] = o.o_t; a.ext_out.d_p[e] = o.d_p; }
         uint32_t s = wave_append(a.sh_out.count, o.kind == 2);
         if (o.kind == 2) { a.sh_out.o_t[s] = o.o_t; a.sh_out.d_p[s] = o.d_p; }
+        if (a.free_list) {           // hand the slot back for regeneration
+            uint32_t f = wave_append(a.free_count, o.done);
+            if (o.done) a.free_list[f] = (uint32_t)pid;
+        }
     }
 }
 
-// Primary rays (pathtracer entry, udpt.cl:162-189) for a list of segments.
+// Primary rays (pathtracer entry, udpt.cl:162-189) for a list of segments:
+// work item -> (segment, pixel), into a path slot.
 __global__ __launch_bounds__(256) void k_generate(GenArgs a, DevCapture cap)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.n_paths) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < a.n_paths;
+    const int item = a.item0 + i;
+    int p = i;
+    if (a.free_list && live) p = (int)a.free_list[a.free_top - 1 - (uint32_t)i];
+    // regeneration appends to the queue behind the rays already in it
+    uint32_t qpos = (uint32_t)i;
+    if (a.free_list) {
+        qpos = wave_append(a.ext_out.count, live);
+        if (i == 0) *a.free_count = a.free_top - (uint32_t)a.n_paths;
+    }
+    if (!live) return;
     int lo = 0, hi = a.n_segs - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (a.segs[mid].path_base <= p) lo = mid; else hi = mid - 1;
+        if (a.segs[mid].path_base <= item) lo = mid; else hi = mid - 1;
     }
     const GenSeg& g = a.segs[lo];
-    int q = p - g.path_base;
+    int q = item - g.path_base;
     int x = g.x0 + q % g.w, y = g.y0 + q / g.w;
     const DevJob& J = a.jobs[g.job];
     const DevScene& S = a.scenes[J.scene];
@@ -409,9 +428,9 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a, DevCapture cap)
     a.P.slot[p] = g.slot;
     a.P.state[p] = st;
     a.P.rd[p] = st4(d);
-    a.ext_out.o_t[p] = make_float4(o.x, o.y, o.z, tmax);
-    a.ext_out.d_p[p] = make_float4(d.x, d.y, d.z, __int_as_float(p));
-    if (p == 0) *a.ext_out.count = (uint32_t)a.n_paths;
+    a.ext_out.o_t[qpos] = make_float4(o.x, o.y, o.z, tmax);
+    a.ext_out.d_p[qpos] = make_float4(d.x, d.y, d.z, __int_as_float(p));
+    if (!a.free_list && i == 0) *a.ext_out.count = (uint32_t)a.n_paths;
     if (cap.prim_tri && g.slot == 0) {
         cap.shadow_occ[y * J.W + x] = -1;
         cap.shadow_d[y * J.W + x] = make_float4(0, 0, 0, 0);

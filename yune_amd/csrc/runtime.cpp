@@ -93,6 +93,8 @@ struct yn_ctx {
     DevJob* d_jobs = nullptr;
     // wavefront buffers
     int64_t cap_paths = 0;
+    int64_t sample_cap = 0;           // samples buffer entries (frames x pixels per batch)
+    uint32_t* free_list = nullptr;    // slots of ended paths, for regeneration
     PathSoA P{};
     RayQueue E[2]{}, Sh[2]{};
     DevCounters* ctr = nullptr;
@@ -192,9 +194,16 @@ int alloc_wavefront(yn_ctx* c, int64_t n)
         r |= A(&c->E[k].o_t); r |= A(&c->E[k].d_p);
         r |= A(&c->Sh[k].o_t); r |= A(&c->Sh[k].d_p);
     }
-    r |= A(&c->samples);
+    r |= A(&c->free_list);
+    {   // a batch may hold more work than paths in flight (regeneration):
+        // its samples buffer is twice the path capacity
+        int rr = dalloc(c, &c->samples, (size_t)n * 2);
+        if (rr == 0) c->allocs.push_back(c->samples);
+        r |= rr;
+    }
     if (r) return r < 0 ? r : YN_ERR_HIP;
     c->cap_paths = n;
+    c->sample_cap = 2 * n;
     return 0;
 }
 
@@ -304,7 +313,12 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     }
     if (total == 0) return 0;
     const bool bdpt = c->jobs[regs[0].job].p.mode == YN_MODE_BDPT;
-    if (total > (bdpt ? c->cap_bdpt : c->cap_paths)) return fail(c, YN_ERR_INVALID, "batch larger than the path capacity");
+    // paths in flight: the whole batch when it fits, else the capacity, with
+    // ended paths' slots regenerated from the remaining work items
+    const int64_t C = bdpt ? c->cap_bdpt : (c->batch_paths > 0 ? std::min(c->batch_paths, c->cap_paths) : c->cap_paths);
+    if (bdpt ? total > C : total > c->sample_cap) return fail(c, YN_ERR_INVALID, "batch larger than the path capacity");
+    const bool regen = !bdpt && total > C;
+    int64_t issued = std::min<int64_t>(total, C);
     if ((int)segs.size() > c->seg_cap) {
         if (c->d_segs) hipFree(c->d_segs);
         c->d_segs = nullptr;
@@ -335,7 +349,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     ga.ext_out = c->E[0];
     ga.segs = c->d_segs;
     ga.n_segs = (int32_t)segs.size();
-    ga.n_paths = (int32_t)total;
+    ga.n_paths = (int32_t)issued;
     if (bdpt) {
         ga.B = c->B;
         ga.list_out = c->lst[0];
@@ -443,6 +457,8 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
                 sa.zero0 = q + 4;
                 sa.zero1 = q + 5;
                 sa.ctr = c->ctr;
+                sa.free_list = regen ? c->free_list : nullptr;
+                sa.free_count = q + 10;
                 launch_shade_pt(sa, cap, shade_grid, c->cfg.shade_waves, s);
             }
             if (timed) { hipEventRecord(e5, s); spans.push_back({e4, e5, 2}); }
@@ -451,9 +467,29 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, s));
         HIPCK(c, hipStreamSynchronize(s));
         int nxt = it & 1;
-        if (bdpt ? c->h_ctr->q[8 + nxt] == 0 : (c->h_ctr->q[nxt] == 0 && c->h_ctr->q[2 + nxt] == 0)) break;
         if (c->h_ctr->q[7]) return fail(c, YN_ERR_INVALID, "traversal stack overflow");
         if (it > 200000) return fail(c, YN_ERR_HIP, "wavefront loop did not drain");
+        bool empty = bdpt ? c->h_ctr->q[8 + nxt] == 0 : (c->h_ctr->q[nxt] == 0 && c->h_ctr->q[2 + nxt] == 0);
+        if (regen && issued < total) {
+            // refill the slots freed so far once there are enough of them (or
+            // the wavefront has drained), appending to the queue the next
+            // iteration reads
+            const uint32_t f = c->h_ctr->q[10];
+            const int64_t need = total - issued;
+            if (f > 0 && ((int64_t)f >= std::min<int64_t>(need, std::max<int64_t>(C / 8, 1)) || empty)) {
+                GenArgs gr = ga;
+                gr.item0 = (int32_t)issued;
+                gr.n_paths = (int32_t)std::min<int64_t>((int64_t)f, need);
+                gr.free_list = c->free_list;
+                gr.free_top = f;
+                gr.free_count = q + 10;
+                gr.ext_out = c->E[nxt];
+                launch_generate(gr, cap, s);
+                issued += gr.n_paths;
+                continue;
+            }
+        }
+        if (empty) break;
     }
     auto t1 = std::chrono::steady_clock::now();
     for (const Span& sp : spans) {
@@ -499,7 +535,8 @@ int render_rect(yn_ctx* c, int jid, float4* accum, int x0, int y0, int w, int h,
         if (int r = ensure_bdpt(c, J.p.bdpt_bounces)) return r;
         pcap = c->cap_bdpt;
     }
-    if (c->batch_paths > 0) pcap = std::min(pcap, c->batch_paths);
+    if (J.p.mode == YN_MODE_PT) pcap = c->sample_cap;                  // regeneration keeps C paths in flight
+    else if (c->batch_paths > 0) pcap = std::min(pcap, c->batch_paths);
     // Sample slots are indexed [slot][W*H]; a batch may hold K frames when
     // K full images fit in the path capacity.
     int64_t kcap = pcap / std::max<int64_t>(full, 1);
@@ -596,7 +633,8 @@ int serve_round(yn_ctx* c, int* frames_out)
         if (int r = ensure_bdpt(c, max_kb)) return r;
         pcap = c->cap_bdpt;
     }
-    if (c->batch_paths > 0) pcap = std::min(pcap, c->batch_paths);
+    if (mode == YN_MODE_PT) pcap = c->sample_cap;
+    else if (c->batch_paths > 0) pcap = std::min(pcap, c->batch_paths);
     // a job whose frame exceeds the batch runs alone, in row bands
     {
         JobRec& J = c->jobs[order[0]];

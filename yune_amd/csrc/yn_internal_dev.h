@@ -44,6 +44,8 @@ struct ShadeArgs {
     uint32_t* zero0;            // work counters to clear for the next traces
     uint32_t* zero1;
     DevCounters* ctr;
+    uint32_t* free_list;        // slots of ended paths (regeneration), or null
+    uint32_t* free_count;
 };
 
 // Shade pass of the bidirectional tracer: advances the paths of `list_in`
@@ -74,6 +76,13 @@ struct GenArgs {
     BdptSoA B;                  // bidirectional mode only
     uint32_t* list_out;         // bidirectional mode: every path starts listed
     uint32_t* list_count;
+    // regeneration: work items [item0, item0 + n_paths) take the slots
+    // free_list[free_top - 1 - i] and append their rays to ext_out; without
+    // a free list, item i takes slot i and ray position i
+    int32_t item0;
+    const uint32_t* free_list;
+    uint32_t free_top;
+    uint32_t* free_count;
 };
 
 struct ResolveArgs {
