@@ -72,8 +72,11 @@ __device__ __forceinline__ Out make_ray(int kind, f4 o, f4 d, float tmax, int p)
     return r;
 }
 
-// One shade step of path p.  `ev_o`, `ev_d`: the ray whose result arrived.
-__device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool from_ext, float4 ev_o, float4 ev_d)
+// One shade step of path p.  The ray whose result arrived is rebuilt from
+// the path state, bit for bit as it was made: a camera ray starts at the
+// camera's eye (createRay, udpt.cl:236) along P.rd; a bounce ray at
+// hp + nd * EPSILON along P.nd (udpt.cl:766, 906, 945).
+__device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
 {
     const PathSoA& P = a.P;
     const int jid = P.job[p];
@@ -128,7 +131,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
             cap.prim_light[pix] = (htri < 0 && (st & F_LCAND)) ? 0 : -1;
             cap.prim_t[pix] = h.x;
         }
-        f4 d = ld(ev_d); d.w = 0.0f;
+        f4 d = rd;
         if (htri < 0) {
             if (st & F_LCAND) {     // udpt.cl:440-446
                 const_color = dot(d, ld(L.normal)) < 0 ? mk(1, 1, 1, 1) : mk(0.1f, .1f, .1f, 1);
@@ -137,7 +140,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
             }
             step = S_FINISH_CONST;
         } else {
-            f4 o = mk(ev_o.x, ev_o.y, ev_o.z, 1.0f);
+            f4 o = mk(J.cam.r1.w, J.cam.r2.w, J.cam.r3.w, 1.0f);
             tri = htri;
             hp = o + d * h.x;
             hn = shade_normal(S.tri_shade, tri, h.z, h.w);
@@ -169,7 +172,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p, bool fr
                 if ((st & F_LCAND) && m.spec) { indirect = indirect + tp * ld(L.ke); ind_dirty = true; }
                 step = S_FINISH;
             } else {
-                f4 o = mk(ev_o.x, ev_o.y, ev_o.z, 1.0f);
+                f4 o = hp + nd * kEps;
                 f4 nhp = o + nd * h.x;
                 f4 nhn = shade_normal(S.tri_shade, htri, h.z, h.w);
                 Mat nm = load_mat(S.mats, S.tri_mat[htri]);
@@ -372,10 +375,8 @@ __global__ __launch_bounds__(256, WAVES) void k_shade_pt(ShadeArgs a, DevCapture
         if (i < total) {
             bool ext = i < n_ext;
             uint32_t k = ext ? i : i - n_ext;
-            float4 ro = ext ? a.ext_in.o_t[k] : a.sh_in.o_t[k];
-            float4 rdp = ext ? a.ext_in.d_p[k] : a.sh_in.d_p[k];
-            pid = __float_as_int(rdp.w);
-            o = advance(a, cap, pid, ext, ro, rdp);
+            pid = __float_as_int((ext ? a.ext_in.d_p[k] : a.sh_in.d_p[k]).w);
+            o = advance(a, cap, pid);
         }
         uint32_t e = wave_append(a.ext_out.count, o.kind == 1);
         if (o.kind == 1) { a.ext_out.o_t[e] = o.o_t; a.ext_out.d_p[e] = o.d_p; }
