@@ -96,6 +96,25 @@ void yn_host_scene_free(yn_host_scene* s);
 int  yn_camera(const float side[3], const float up[3], const float look_at[3],
                const float eye[3], float yfov_deg, yn_cam* out);
 
+/* ---- camera controls: Camera (src/Camera.cpp:36-166) without glm.  The
+ * state holds the basis (side, up, look_at), the eye and the view-to-world
+ * matrix (columns side, up, -look_at, eye).  yn_camctl_orient is
+ * Camera::setOrientation: a step of move_speed along +-look_at, +-side or
+ * +-up (the first non-zero component of dir, tested z, x, y), then a pitch
+ * of pitch * rotation_speed radians about the camera's side axis (skipped
+ * when it would turn `up` below the horizon) and a yaw of
+ * yaw * rotation_speed about world +Y. ---- */
+typedef struct yn_camctl {
+    yn_float4 side, up, look_at, eye;
+    yn_float4 view2world[4];        /* columns */
+    float y_fov, rotation_speed, move_speed, view_plane_dist;
+} yn_camctl;
+int  yn_camctl_init(float y_fov, float rotation_speed, float move_speed, yn_camctl* out);  /* identity view */
+int  yn_camctl_set_view(yn_camctl* c, const float side[3], const float up[3], const float look_at[3],
+                        const float eye[3]);
+int  yn_camctl_orient(yn_camctl* c, const float dir[3], float pitch, float yaw);
+int  yn_camctl_pack(const yn_camctl* c, yn_cam* out);                                     /* Camera::setBuffer */
+
 /* ---- scenes on the device: CLManager::setupVertexBuffer / setupMatBuffer /
  *      setupBVHBuffer (src/CLManager.cpp:420-484), kernel args 3-7 ----
  * n_node == 0 selects brute force over all triangles (arg 6 == 0,

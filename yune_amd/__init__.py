@@ -21,8 +21,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import (MAT_DTYPE, NODE_DTYPE, TRI_DTYPE, YN_MODE_BDPT, YN_MODE_PT, Cam, Config, HostScene, JobStatus,
-                   TuneResult,
+from ._lib import (MAT_DTYPE, NODE_DTYPE, TRI_DTYPE, YN_MODE_BDPT, YN_MODE_PT, Cam, CamCtl, Config, HostScene,
+                   JobStatus, TuneResult,
                    LightQuad, RenderParams, SceneInfo, Stats, YuneError, check, lib, ptr)
 
 __all__ = ["TRI_DTYPE", "MAT_DTYPE", "NODE_DTYPE", "YN_MODE_PT", "YN_MODE_BDPT", "YuneError", "Scene",
@@ -110,6 +110,32 @@ def camera(side, up, look_at, eye, yfov: float = 60.0) -> Cam:
     fp = C.POINTER(C.c_float)
     check(lib().yn_camera(*[a.ctypes.data_as(fp) for a in arrs], float(yfov), C.byref(c)))
     return c
+
+
+class CameraControl:
+    """Camera (src/Camera.cpp): move / pitch / yaw controls and setBuffer."""
+
+    def __init__(self, y_fov: float = 60.0, rotation_speed: float = 0.25, move_speed: float = 0.1):
+        self.s = CamCtl()
+        check(lib().yn_camctl_init(y_fov, rotation_speed, move_speed, C.byref(self.s)))
+
+    def set_view(self, side, up, look_at, eye):
+        arrs = [np.ascontiguousarray(v, dtype=np.float32) for v in (side, up, look_at, eye)]
+        fp = C.POINTER(C.c_float)
+        check(lib().yn_camctl_set_view(C.byref(self.s), *[a.ctypes.data_as(fp) for a in arrs]))
+
+    def orient(self, direction=(0.0, 0.0, 0.0), pitch: float = 0.0, yaw: float = 0.0):
+        d = np.ascontiguousarray(direction, dtype=np.float32)
+        check(lib().yn_camctl_orient(C.byref(self.s), d.ctypes.data_as(C.POINTER(C.c_float)), pitch, yaw))
+
+    def cam(self) -> Cam:
+        c = Cam()
+        check(lib().yn_camctl_pack(C.byref(self.s), C.byref(c)))
+        return c
+
+    def basis(self):
+        g = lambda v: np.array(list(v), np.float32)  # noqa: E731
+        return g(self.s.side), g(self.s.up), g(self.s.look_at), g(self.s.eye)
 
 
 def light_quad(pos, normal, ke, edge_l, edge_w) -> LightQuad:

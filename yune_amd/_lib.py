@@ -67,6 +67,11 @@ class SceneInfo(C.Structure):
                 ("tri_bytes", C.c_size_t), ("shade_bytes", C.c_size_t)]
 
 
+class CamCtl(C.Structure):
+    _fields_ = [("side", F4), ("up", F4), ("look_at", F4), ("eye", F4), ("view2world", F4 * 4), ("y_fov", C.c_float),
+                ("rotation_speed", C.c_float), ("move_speed", C.c_float), ("view_plane_dist", C.c_float)]
+
+
 class TuneResult(C.Structure):
     _fields_ = [("bvh_width", C.c_int32), ("trace_block", C.c_int32), ("trace_chunk", C.c_int32),
                 ("lds_stack", C.c_int32), ("shade_waves", C.c_int32), ("n_tried", C.c_int32),
@@ -134,6 +139,10 @@ SIGNATURES = {
     "yn_fmath_eval": (I32, [P, I32, P, P, I32, P]),
     "yn_autotune": (I32, [P, I32, C.POINTER(Cam), I32, I32, C.POINTER(RenderParams), I32, C.POINTER(TuneResult)]),
     "yn_config_get": (I32, [P, C.POINTER(Config), C.POINTER(C.c_int64)]),
+    "yn_camctl_init": (I32, [C.c_float, C.c_float, C.c_float, C.POINTER(CamCtl)]),
+    "yn_camctl_set_view": (I32, [C.POINTER(CamCtl), PF, PF, PF, PF]),
+    "yn_camctl_orient": (I32, [C.POINTER(CamCtl), PF, C.c_float, C.c_float]),
+    "yn_camctl_pack": (I32, [C.POINTER(CamCtl), C.POINTER(Cam)]),
 }
 
 _lib = None
