@@ -20,6 +20,8 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "../../include/yune_hip.h"
 #include "yn_internal_dev.h"
 
@@ -95,6 +97,10 @@ struct yn_ctx {
     int64_t cap_paths = 0;
     int64_t sample_cap = 0;           // samples buffer entries (frames x pixels per batch)
     uint32_t* free_list = nullptr;    // slots of ended paths, for regeneration
+    uint32_t* free_alt = nullptr;     // the free list sorted by slot (double buffer)
+    void* sort_tmp = nullptr;         // radix-sort workspace
+    size_t sort_tmp_bytes = 0;
+    int slot_bits = 1;                // bits of a slot index
     PathSoA P{};
     RayQueue E[2]{}, Sh[2]{};
     DevCounters* ctr = nullptr;
@@ -195,6 +201,16 @@ int alloc_wavefront(yn_ctx* c, int64_t n)
         r |= A(&c->Sh[k].o_t); r |= A(&c->Sh[k].d_p);
     }
     r |= A(&c->free_list);
+    r |= A(&c->free_alt);
+    if (r == 0) {
+        c->slot_bits = 1;
+        while (((int64_t)1 << c->slot_bits) < n) ++c->slot_bits;
+        size_t tb = 0;
+        hipcub::DeviceRadixSort::SortKeys(nullptr, tb, c->free_list, c->free_alt, (int)n, 0, c->slot_bits, c->stream);
+        char* t = nullptr;
+        r |= dalloc(c, &t, std::max<size_t>(tb, 16));
+        if (t) { c->allocs.push_back(t); c->sort_tmp = t; c->sort_tmp_bytes = tb; }
+    }
     {   // a batch may hold more work than paths in flight (regeneration):
         // its samples buffer is twice the path capacity
         int rr = dalloc(c, &c->samples, (size_t)n * 2);
@@ -477,6 +493,12 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             const uint32_t f = c->h_ctr->q[10];
             const int64_t need = total - issued;
             if (f > 0 && ((int64_t)f >= std::min<int64_t>(need, std::max<int64_t>(C / 8, 1)) || empty)) {
+                // slots come back in the order paths ended; sorted, the new
+                // paths take ascending slots and their state stays coalesced
+                size_t tb = c->sort_tmp_bytes;
+                HIPCK(c, hipcub::DeviceRadixSort::SortKeys(c->sort_tmp, tb, c->free_list, c->free_alt, (int)f, 0,
+                                                           c->slot_bits, s));
+                std::swap(c->free_list, c->free_alt);
                 GenArgs gr = ga;
                 gr.item0 = (int32_t)issued;
                 gr.n_paths = (int32_t)std::min<int64_t>((int64_t)f, need);
