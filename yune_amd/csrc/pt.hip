@@ -79,7 +79,8 @@ __device__ __forceinline__ Out make_ray(int kind, f4 o, f4 d, float tmax, int p)
 __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
 {
     const PathSoA& P = a.P;
-    const int jid = P.job[p];
+    const int4 m0 = P.g0[2 * (size_t)p], m1 = P.g0[2 * (size_t)p + 1];
+    const int jid = m1.y;
     const DevJob& J = a.jobs[jid];
     const yn_render_params& RP = J.p;
     const DevScene& S = a.scenes[J.scene];
@@ -87,15 +88,18 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
     const bool on = RP.oren_nayar != 0;
     const bool fix = RP.fix_mis != 0;
 
-    int st = P.state[p];
+    int st = m0.y;
     int code = st & ST_MASK;
-    uint32_t seed = P.seed[p];
-    const int pix = P.pix[p];
-    const int slot = P.slot[p];
+    uint32_t seed = (uint32_t)m0.x;
+    const int pix = m1.x;
+    const int slot = m1.z;
     const bool cap_on = cap.prim_tri != nullptr && slot == 0;
 
-    int tri = P.tri[p];
-    f4 hp = ld(P.hp[p]), hn = ld(P.hn[p]), rd = ld(P.rd[p]);
+    int tri = m0.w;
+    const size_t r4 = 4 * (size_t)p;
+    f4 hp = ld(P.g1[r4]), hn = ld(P.g1[r4 + 1]), rd = ld(P.g1[r4 + 2]);
+    float nee_prob = __int_as_float(m1.w);
+    bool np_dirty = false;
     // throughput and radiance sums are read only by the steps that use them
     // and written back only when changed (the shade pass is bound by this
     // path-state traffic)
@@ -103,13 +107,13 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
     bool acc_loaded = false, tp_dirty = false, dir_dirty = false, ind_dirty = false, v_dirty = false;
     auto load_acc = [&]() {
         if (!acc_loaded) {
-            tp = ld(P.tp[p]);
-            direct = ld(P.direct[p]);
-            indirect = ld(P.indirect[p]);
+            tp = ld(P.g1[r4 + 3]);
+            direct = ld(P.g2[r4]);
+            indirect = ld(P.g2[r4 + 1]);
             acc_loaded = true;
         }
     };
-    int bounce = P.bounce[p];
+    int bounce = m0.z;
 
     // NEE scratch
     f4 wi = mk(0, 0, 0, 0);
@@ -165,8 +169,8 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
             float4 h = P.hit[p];
             int htri = __float_as_int(h.y);
             Mat m = load_mat(S.mats, S.tri_mat[tri]);
-            f4 nd = ld(P.nd[p]);
-            float4 bnc = P.bnc[p];
+            f4 nd = ld(P.g2[r4 + 2]);
+            float4 bnc = P.g2[r4 + 3];
             load_acc();
             if (htri < 0) {
                 if ((st & F_LCAND) && m.spec) { indirect = indirect + tp * ld(L.ke); ind_dirty = true; }
@@ -228,18 +232,19 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
                 cap.shadow_d[pix] = make_float4(wi.x, wi.y, wi.z, 1.0f);
             }
             if (quad_hit(L, so, wi, len, tq)) { occ = 1; step = S_NEE_SHADOW; continue; }
-            P.wi[p] = make_float4(wi.x, wi.y, wi.z, lpdf);
-            P.nee_prob[p] = nprob;
+            P.g3[r4] = make_float4(wi.x, wi.y, wi.z, lpdf);
+            nee_prob = nprob;
+            np_dirty = true;
             st = (st & ~ST_MASK) | ST_NEE_SH;
             out = make_ray(2, so, wi, len, p);
             break;
         }
         if (step == S_NEE_SHADOW) {  // udpt.cl:557-565 and the MIS light weight
             if (code == ST_NEE_SH) {
-                float4 w4 = P.wi[p];
+                float4 w4 = P.g3[r4];
                 wi = mk(w4.x, w4.y, w4.z, 0.0f);
                 lpdf = w4.w;
-                nprob = P.nee_prob[p];
+                nprob = nee_prob;
                 code = ST_DONE;     // consumed
             }
             if (cap_on && (st & F_DIRECT) && !(st & F_CAPSH)) { cap.shadow_occ[pix] = occ; st |= F_CAPSH; }
@@ -266,10 +271,11 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
             float tq;
             if (!quad_hit(L, bo, bd, INFINITY, tq)) { nee_res = ls + em; step = S_NEE_DONE; continue; }
             st = glossy ? (st | F_NGLOSSY) : (st & ~F_NGLOSSY);
-            P.ls[p] = st4(ls);
-            P.mis[p] = make_float4(bd.x, bd.y, bd.z, bpdf);
-            P.wi[p] = make_float4(wi.x, wi.y, wi.z, lpdf);
-            P.nee_prob[p] = nprob;
+            P.g3[r4 + 1] = st4(ls);
+            P.g3[r4 + 2] = make_float4(bd.x, bd.y, bd.z, bpdf);
+            P.g3[r4] = make_float4(wi.x, wi.y, wi.z, lpdf);
+            nee_prob = nprob;
+            np_dirty = true;
             st = (st & ~ST_MASK) | ST_NEE_MIS;
             out = make_ray(2, bo, bd, tq, p);
             break;
@@ -277,13 +283,13 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
         if (step == S_NEE_MIS) {     // udpt.cl:599-607
             Mat m = load_mat(S.mats, S.tri_mat[tri]);
             em = m.ke;
-            ls = ld(P.ls[p]);
+            ls = ld(P.g3[r4 + 1]);
             if (occ) { nee_res = ls + em; step = S_NEE_DONE; continue; }
-            float4 mb = P.mis[p];
+            float4 mb = P.g3[r4 + 2];
             f4 bd = mk(mb.x, mb.y, mb.z, 0.0f);
             float bpdf = mb.w;
-            lpdf = P.wi[p].w;
-            nprob = P.nee_prob[p];
+            lpdf = P.g3[r4].w;
+            nprob = nee_prob;
             bool glossy = (st & F_NGLOSSY) != 0;
             float mw = power_heuristic(bpdf, lpdf, bpdf);
             f4 bs = (eval_brdf(m, bd, -rd, hp, hn, glossy, nprob, on, false) * ld(L.ke)) * fmaxf(dot(bd, hn), 0.0f);
@@ -317,8 +323,8 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
             st &= ~F_LCAND;
             if (quad_hit(L, no, nd, INFINITY, tq)) { tmax = tq; st |= F_LCAND; }
             st = glossy ? (st | F_BGLOSSY) : (st & ~F_BGLOSSY);
-            P.nd[p] = st4(nd);
-            P.bnc[p] = make_float4(pdf, bprob, ior, 0.0f);
+            P.g2[r4 + 2] = st4(nd);
+            P.g2[r4 + 3] = make_float4(pdf, bprob, ior, 0.0f);
             st = (st & ~ST_MASK) | ST_BOUNCE;
             out = make_ray(1, no, nd, tmax, p);
             break;
@@ -338,19 +344,18 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
     }
     out.done = out.kind == 0;
 
-    P.state[p] = st;
-    P.seed[p] = seed;
+    P.g0[2 * (size_t)p] = make_int4((int)seed, st, out.kind != 0 && v_dirty ? bounce : m0.z,
+                                    out.kind != 0 && v_dirty ? tri : m0.w);
     if (out.kind != 0) {
+        if (np_dirty) P.g0[2 * (size_t)p + 1] = make_int4(m1.x, m1.y, m1.z, __float_as_int(nee_prob));
         if (v_dirty) {
-            P.tri[p] = tri;
-            P.bounce[p] = bounce;
-            P.hp[p] = st4(hp);
-            P.hn[p] = st4(hn);
-            P.rd[p] = st4(rd);
+            P.g1[r4] = st4(hp);
+            P.g1[r4 + 1] = st4(hn);
+            P.g1[r4 + 2] = st4(rd);
         }
-        if (tp_dirty) P.tp[p] = st4(tp);
-        if (dir_dirty) P.direct[p] = st4(direct);
-        if (ind_dirty) P.indirect[p] = st4(indirect);
+        if (tp_dirty) P.g1[r4 + 3] = st4(tp);
+        if (dir_dirty) P.g2[r4] = st4(direct);
+        if (ind_dirty) P.g2[r4 + 1] = st4(indirect);
     }
     return out;
 }
@@ -423,12 +428,10 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a, DevCapture cap)
     float tq, tmax = INFINITY;
     int st = ST_PRIMARY;
     if (quad_hit(S.light, o, d, INFINITY, tq)) { tmax = tq; st |= F_LCAND; }
-    a.P.seed[p] = seed;
-    a.P.pix[p] = y * J.W + x;
-    a.P.job[p] = g.job;
-    a.P.slot[p] = g.slot;
-    a.P.state[p] = st;
-    a.P.rd[p] = st4(d);
+    a.P.g0[2 * (size_t)p] = make_int4((int)seed, st, 0, -1);
+    a.P.g0[2 * (size_t)p + 1] = make_int4(y * J.W + x, g.job, g.slot, 0);
+    a.P.job[p] = g.job;                 // read by the trace kernels' scene lookup
+    a.P.g1[4 * (size_t)p + 2] = st4(d);
     a.ext_out.o_t[qpos] = make_float4(o.x, o.y, o.z, tmax);
     a.ext_out.d_p[qpos] = make_float4(d.x, d.y, d.z, __int_as_float(p));
     if (!a.free_list && i == 0) *a.ext_out.count = (uint32_t)a.n_paths;

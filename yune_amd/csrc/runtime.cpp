@@ -196,6 +196,14 @@ int alloc_wavefront(yn_ctx* c, int64_t n)
     r |= A(&P.hp); r |= A(&P.hn); r |= A(&P.rd); r |= A(&P.tp); r |= A(&P.direct); r |= A(&P.indirect);
     r |= A(&P.nd); r |= A(&P.bnc); r |= A(&P.wi); r |= A(&P.ls); r |= A(&P.mis); r |= A(&P.nee_prob);
     r |= A(&P.hit); r |= A(&P.occ);
+    {
+        auto B = [&](auto** q, size_t k) -> int {
+            int rr = dalloc(c, q, (size_t)n * k);
+            if (rr == 0) c->allocs.push_back(*q);
+            return rr;
+        };
+        r |= B(&P.g0, 2); r |= B(&P.g1, 4); r |= B(&P.g2, 4); r |= B(&P.g3, 4);
+    }
     for (int k = 0; k < 2; ++k) {
         r |= A(&c->E[k].o_t); r |= A(&c->E[k].d_p);
         r |= A(&c->Sh[k].o_t); r |= A(&c->Sh[k].d_p);

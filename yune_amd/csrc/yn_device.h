@@ -70,6 +70,13 @@ struct PathSoA {
     float* nee_prob;            // NEE: lobe probability of the light sample
     float4* hit;                // closest-hit result: t, tri bits, u, v
     int32_t* occ;               // any-hit result
+    // Unidirectional shading state, grouped by use: a shade step reads and
+    // writes whole 32/64-byte records instead of one cache line per field,
+    // which matters because a queue's paths are in no particular slot order.
+    int4* g0;                   // [2 per path] {seed, state, bounce, tri}, {pix, job, slot, nee_prob bits}
+    float4* g1;                 // [4 per path] hp, hn, rd, tp
+    float4* g2;                 // [4 per path] direct, indirect, nd, bnc
+    float4* g3;                 // [4 per path] wi (w: light pdf), ls, mis, -
 };
 
 // Bidirectional paths (bdpt.cl): both subpaths' vertices plus the
