@@ -274,7 +274,7 @@ def test_regeneration_with_work_beyond_capacity():
     s = Y.gen_cornell()
     W, H, F = 64, 64, 16
     kw = dict(mis=1, max_bounces=8, rr_threshold=2)
-    with Y.Context(max_paths=1 << 12) as c:
+    with Y.Context(max_paths=1 << 12, batch_factor=2) as c:
         job = _cornell_job(c, s, W, H, Y.params_reference(**kw), 21)
         job.render(F)
         gimg = job.read()

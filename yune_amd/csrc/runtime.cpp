@@ -565,8 +565,9 @@ int render_rect(yn_ctx* c, int jid, float4* accum, int x0, int y0, int w, int h,
         if (int r = ensure_bdpt(c, J.p.bdpt_bounces)) return r;
         pcap = c->cap_bdpt;
     }
-    if (J.p.mode == YN_MODE_PT) pcap = c->sample_cap;                  // regeneration keeps C paths in flight
-    else if (c->batch_paths > 0) pcap = std::min(pcap, c->batch_paths);
+    if (c->batch_paths > 0) pcap = std::min(pcap, c->batch_paths);
+    // with regeneration a batch holds more work than paths in flight
+    if (J.p.mode == YN_MODE_PT) pcap = std::min<int64_t>(c->sample_cap, pcap * c->cfg.batch_factor);
     // Sample slots are indexed [slot][W*H]; a batch may hold K frames when
     // K full images fit in the path capacity.
     int64_t kcap = pcap / std::max<int64_t>(full, 1);
@@ -663,8 +664,8 @@ int serve_round(yn_ctx* c, int* frames_out)
         if (int r = ensure_bdpt(c, max_kb)) return r;
         pcap = c->cap_bdpt;
     }
-    if (mode == YN_MODE_PT) pcap = c->sample_cap;
-    else if (c->batch_paths > 0) pcap = std::min(pcap, c->batch_paths);
+    if (c->batch_paths > 0) pcap = std::min(pcap, c->batch_paths);
+    if (mode == YN_MODE_PT) pcap = std::min<int64_t>(c->sample_cap, pcap * c->cfg.batch_factor);
     // a job whose frame exceeds the batch runs alone, in row bands
     {
         JobRec& J = c->jobs[order[0]];
@@ -859,6 +860,8 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.lds_stack != 8 && k.lds_stack != 16 && k.lds_stack != 32)
         return fail(nullptr, YN_ERR_INVALID, "lds_stack must be 8, 16 or 32");
     if (k.shade_waves < 2 || k.shade_waves > 4) return fail(nullptr, YN_ERR_INVALID, "shade_waves must be 2, 3 or 4");
+    if (k.batch_factor == 0) k.batch_factor = 1;
+    if (k.batch_factor != 1 && k.batch_factor != 2) return fail(nullptr, YN_ERR_INVALID, "batch_factor must be 1 or 2");
     if (k.bvh_width != 2 && k.bvh_width != 4) return fail(nullptr, YN_ERR_INVALID, "bvh_width must be 2 or 4");
     if (k.trace_block != 64 && k.trace_block != 128 && k.trace_block != 256)
         return fail(nullptr, YN_ERR_INVALID, "trace_block must be 64, 128 or 256");
@@ -1276,6 +1279,7 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         {3, {16, 8, 32}},                       // LDS stack entries
         {4, {2, 3, 4}},                         // shade occupancy
         {5, {cap, cap / 2, cap / 4}},           // paths per batch
+        {6, {1, 2}},                            // work per batch over paths in flight (regeneration)
     };
     auto get = [&](int kind) -> int64_t {
         switch (kind) {
@@ -1284,7 +1288,8 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         case 2: return c->cfg.trace_chunk;
         case 3: return c->cfg.lds_stack;
         case 4: return c->cfg.shade_waves;
-        default: return c->batch_paths > 0 ? c->batch_paths : cap;
+        case 5: return c->batch_paths > 0 ? c->batch_paths : cap;
+        default: return c->cfg.batch_factor;
         }
     };
     auto set = [&](int kind, int64_t v) -> int {
@@ -1294,7 +1299,8 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         case 2: c->cfg.trace_chunk = (int)v; return 0;
         case 3: c->cfg.lds_stack = (int)v; return 0;
         case 4: c->cfg.shade_waves = (int)v; return 0;
-        default: c->batch_paths = v; return 0;
+        case 5: c->batch_paths = v; return 0;
+        default: c->cfg.batch_factor = (int)v; return 0;
         }
     };
     double best = measure();
@@ -1321,6 +1327,7 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
     out->lds_stack = c->cfg.lds_stack;
     out->shade_waves = c->cfg.shade_waves;
     out->batch_paths = c->batch_paths > 0 ? c->batch_paths : cap;
+    out->batch_factor = c->cfg.batch_factor;
     out->mrays_per_s = best / 1e6;
     out->n_tried = tried;
     return 0;
