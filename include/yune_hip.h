@@ -61,6 +61,8 @@ typedef struct yn_config {
     int32_t batch_factor;     /* path-tracing work per batch over paths in flight:
                                  1 (0: default) or 2, when ended paths' slots are
                                  regenerated from the remaining work               */
+    int32_t trace_overlap;    /* 1: each iteration's closest-hit and any-hit traces
+                                 run on two streams and overlap; 0: one stream     */
 } yn_config;
 
 /* Parameters of udpt.cl / bdpt.cl as shipped: GI on, MIS off, no bounce
@@ -208,14 +210,14 @@ double yn_clock_ms(yn_ctx* ctx);                     /* ms since yn_create */
  * timed runs of the given view (frames per run) and keeps, knob by knob,
  * the fastest BVH width (the scenes' GPU layout is rebuilt), trace block
  * size, rays per queue fetch, LDS stack depth, shade occupancy, paths
- * per batch and batch_factor in the context's configuration.  Not while
+ * per batch, batch_factor and trace_overlap in the context's configuration.  Not while
  * served jobs are pending.  yn_config_get reads the configuration in effect. ---- */
 typedef struct yn_tune_result {
     int32_t bvh_width, trace_block, trace_chunk, lds_stack, shade_waves, n_tried;
     int64_t batch_paths;      /* paths in flight per batch                              */
     int32_t batch_factor;     /* work items per batch over paths in flight; 2: ended
                                  paths' slots are regenerated from the remaining work */
-    int32_t pad;
+    int32_t trace_overlap;    /* closest-hit and any-hit traces on two streams        */
     double  mrays_per_s;      /* the chosen configuration on the tuning view */
 } yn_tune_result;
 int  yn_autotune(yn_ctx* ctx, int32_t scene_id, const yn_cam* cam, int32_t width, int32_t height,

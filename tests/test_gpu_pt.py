@@ -283,3 +283,22 @@ def test_regeneration_with_work_beyond_capacity():
     assert st.samples == W * H * F
     assert np.array_equal(gimg[..., 3], oimg[..., 3])
     assert identical(gimg, oimg) >= 0.999
+
+
+@pytest.mark.parametrize("mode", [Y.YN_MODE_PT, Y.YN_MODE_BDPT])
+def test_trace_overlap_keeps_images(mode):
+    """Closest-hit and any-hit traces on two streams (each with its own half
+    of the stack-overflow area, joined before the shade) give the same image
+    as on one stream, bit for bit."""
+    s = Y.gen_icosphere_room(3, 9, 2)
+    W, H, F = 64, 48, 3
+    kw = dict(bdpt_bounces=4) if mode else dict(mis=1, max_bounces=8, rr_threshold=2)
+    p = Y.params_reference(mode, **kw)
+    imgs = []
+    for ov in (0, 1):
+        with Y.Context(max_paths=1 << 14, trace_overlap=ov, lds_stack=8) as c:
+            assert c.config()["trace_overlap"] == ov
+            job = c.job(c.scene(s), s.cam, W, H, p, base_seed=17)
+            job.render(F)
+            imgs.append(job.read())
+    assert np.array_equal(imgs[0].view(np.int32), imgs[1].view(np.int32))

@@ -87,6 +87,8 @@ struct yn_ctx {
     std::string name;
     size_t hbm = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;    // any-hit traces when they overlap the closest-hit ones
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     std::string err;
     yn_config cfg{};
     std::vector<SceneRec> scenes;
@@ -400,7 +402,10 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     int block = c->cfg.trace_block;
     int grid = c->cfg.trace_grid > 0 ? c->cfg.trace_grid : c->n_cu * (2048 / block);
     int shade_grid = c->n_cu * 8;
-    if (int r = ensure_overflow(c, max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * block)) return r;
+    const bool overlap = c->cfg.trace_overlap != 0;
+    // overlapping traces get disjoint halves of the stack overflow area
+    if (int r = ensure_overflow(c, max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * block * (overlap ? 2 : 1)))
+        return r;
     bool count = stat_job.counting != 0;
 
     int ev_base = 0;
@@ -433,6 +438,13 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             ta.zero0 = c->E[nxt].count;
             ta.zero1 = SQ[nxt].count;
             bool timed = (int)spans.size() + 3 < kEvPool / 2;
+            // the any-hit trace depends only on the previous shade, like the
+            // closest-hit one: with overlap it runs on the second stream
+            hipStream_t s2 = overlap ? c->stream2 : s;
+            if (overlap) {
+                HIPCK(c, hipEventRecord(c->fork_ev, s));
+                HIPCK(c, hipStreamWaitEvent(s2, c->fork_ev, 0));
+            }
             hipEvent_t e0 = timed ? ev() : nullptr, e1 = timed ? ev() : nullptr;
             if (timed) hipEventRecord(e0, s);
             if (launch_trace(ta, false, width, max_depth, block, grid, count, c->cfg.lds_stack, s) != 0)
@@ -446,11 +458,19 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             ta.work = q + 5;
             ta.zero0 = bdpt ? q + 8 + nxt : nullptr;    // the next path list
             ta.zero1 = nullptr;
+            if (overlap) {          // the second half of the overflow area
+                ta.ovf_ref = c->ovf_ref + (size_t)c->ovf_depth * grid * block;
+                ta.ovf_t = c->ovf_t + (size_t)c->ovf_depth * grid * block;
+            }
             hipEvent_t e2 = timed ? ev() : nullptr, e3 = timed ? ev() : nullptr;
-            if (timed) hipEventRecord(e2, s);
-            launch_trace(ta, true, width, max_depth, block, grid, count, c->cfg.lds_stack, s);
+            if (timed) hipEventRecord(e2, s2);
+            launch_trace(ta, true, width, max_depth, block, grid, count, c->cfg.lds_stack, s2);
             ta.id_shift = 0;
-            if (timed) { hipEventRecord(e3, s); spans.push_back({e2, e3, 1}); }
+            if (timed) { hipEventRecord(e3, s2); spans.push_back({e2, e3, 1}); }
+            if (overlap) {
+                HIPCK(c, hipEventRecord(c->join_ev, s2));
+                HIPCK(c, hipStreamWaitEvent(s, c->join_ev, 0));
+            }
             // shade
             hipEvent_t e4 = timed ? ev() : nullptr, e5 = timed ? ev() : nullptr;
             if (timed) hipEventRecord(e4, s);
@@ -862,6 +882,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.shade_waves < 2 || k.shade_waves > 4) return fail(nullptr, YN_ERR_INVALID, "shade_waves must be 2, 3 or 4");
     if (k.batch_factor == 0) k.batch_factor = 1;
     if (k.batch_factor != 1 && k.batch_factor != 2) return fail(nullptr, YN_ERR_INVALID, "batch_factor must be 1 or 2");
+    if (k.trace_overlap != 0 && k.trace_overlap != 1) return fail(nullptr, YN_ERR_INVALID, "trace_overlap must be 0 or 1");
     if (k.bvh_width != 2 && k.bvh_width != 4) return fail(nullptr, YN_ERR_INVALID, "bvh_width must be 2 or 4");
     if (k.trace_block != 64 && k.trace_block != 128 && k.trace_block != 256)
         return fail(nullptr, YN_ERR_INVALID, "trace_block must be 64, 128 or 256");
@@ -882,6 +903,11 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     int r = 0;
     auto bail = [&](int code) { int rr = code; yn_destroy(c); return rr; };
     if (hipSetDevice(k.device) != hipSuccess) { g_err = "hipSetDevice failed"; delete c; return YN_ERR_HIP; }
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
+        g_err = "stream setup failed"; delete c; return YN_ERR_HIP;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_err = "hipStreamCreate failed"; delete c; return YN_ERR_HIP;
     }
@@ -913,6 +939,9 @@ void yn_destroy(yn_ctx* c)
     if (c->h_ctr) hipHostFree(c->h_ctr);
     for (int i = 0; i < kEvPool; ++i) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream2) hipStreamDestroy(c->stream2);
+    if (c->fork_ev) hipEventDestroy(c->fork_ev);
+    if (c->join_ev) hipEventDestroy(c->join_ev);
     delete c;
 }
 
@@ -1280,6 +1309,7 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         {4, {2, 3, 4}},                         // shade occupancy
         {5, {cap, cap / 2, cap / 4}},           // paths per batch
         {6, {1, 2}},                            // work per batch over paths in flight (regeneration)
+        {7, {0, 1}},                            // overlapping closest-hit and any-hit traces
     };
     auto get = [&](int kind) -> int64_t {
         switch (kind) {
@@ -1289,7 +1319,8 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         case 3: return c->cfg.lds_stack;
         case 4: return c->cfg.shade_waves;
         case 5: return c->batch_paths > 0 ? c->batch_paths : cap;
-        default: return c->cfg.batch_factor;
+        case 6: return c->cfg.batch_factor;
+        default: return c->cfg.trace_overlap;
         }
     };
     auto set = [&](int kind, int64_t v) -> int {
@@ -1300,7 +1331,8 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         case 3: c->cfg.lds_stack = (int)v; return 0;
         case 4: c->cfg.shade_waves = (int)v; return 0;
         case 5: c->batch_paths = v; return 0;
-        default: c->cfg.batch_factor = (int)v; return 0;
+        case 6: c->cfg.batch_factor = (int)v; return 0;
+        default: c->cfg.trace_overlap = (int)v; return 0;
         }
     };
     double best = measure();
@@ -1328,6 +1360,7 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
     out->shade_waves = c->cfg.shade_waves;
     out->batch_paths = c->batch_paths > 0 ? c->batch_paths : cap;
     out->batch_factor = c->cfg.batch_factor;
+    out->trace_overlap = c->cfg.trace_overlap;
     out->mrays_per_s = best / 1e6;
     out->n_tried = tried;
     return 0;
