@@ -63,6 +63,9 @@ typedef struct yn_config {
                                  regenerated from the remaining work               */
     int32_t trace_overlap;    /* 1: each iteration's closest-hit and any-hit traces
                                  run on two streams and overlap; 0: one stream     */
+    int32_t graph;            /* 1: groups of bounce iterations are captured once in
+                                 a HIP graph and replayed; 0: launched one by one  */
+    int32_t pad;
 } yn_config;
 
 /* Parameters of udpt.cl / bdpt.cl as shipped: GI on, MIS off, no bounce
@@ -150,6 +153,11 @@ int  yn_pathtracer(yn_ctx* ctx, float* out_img, const float* in_img, const yn_ca
 /* tonemap(current, prev, out, reset) (kernels/post-proc/tonemap.cl:14-35) on
  * device images. */
 int  yn_tonemap(yn_ctx* ctx, const float* in_img, float* out_img, int32_t width, int32_t height);
+/* The same with the operator chosen: the Reinhard operator the reference
+ * ships (tonemapReinhard, tonemap.cl:42-47, lum_white 1) or John Hable's
+ * curve (tonemapJohnHable, tonemap.cl:49-58, commented out at :29). */
+enum { YN_TONEMAP_REINHARD = 0, YN_TONEMAP_HABLE = 1 };
+int  yn_tonemap_op(yn_ctx* ctx, const float* in_img, float* out_img, int32_t width, int32_t height, int32_t op);
 
 /* ---- progressive jobs: RendererCore's frame loop (src/RendererCore.cpp:
  * 248-469, 507-579) without the GUI.  Frame f uses rand = f-th output of
@@ -160,6 +168,9 @@ int  yn_job_create(yn_ctx* ctx, int32_t scene_id, const yn_cam* cam, int32_t wid
 int  yn_job_render(yn_ctx* ctx, int32_t job_id, int32_t n_frames);
 int  yn_job_read(yn_ctx* ctx, int32_t job_id, float* rgba);           /* W*H*4 floats  */
 int  yn_job_read_tonemapped(yn_ctx* ctx, int32_t job_id, float* rgba);
+/* Operator used by yn_job_read_tonemapped and the .png of yn_job_save
+ * (default YN_TONEMAP_REINHARD, as the shipped tonemap.cl). */
+int  yn_job_set_tonemap(yn_ctx* ctx, int32_t job_id, int32_t op);
 /* RendererCore::saveImage (src/RendererCore.cpp:608-646): .hdr writes the
  * linear image, .png the tonemapped one as 8-bit, both flipped vertically. */
 int  yn_job_save(yn_ctx* ctx, int32_t job_id, const char* path);
@@ -210,7 +221,7 @@ double yn_clock_ms(yn_ctx* ctx);                     /* ms since yn_create */
  * timed runs of the given view (frames per run) and keeps, knob by knob,
  * the fastest BVH width (the scenes' GPU layout is rebuilt), trace block
  * size, rays per queue fetch, LDS stack depth, shade occupancy, paths
- * per batch, batch_factor and trace_overlap in the context's configuration.  Not while
+ * per batch, batch_factor, trace_overlap and graph in the context's configuration.  Not while
  * served jobs are pending.  yn_config_get reads the configuration in effect. ---- */
 typedef struct yn_tune_result {
     int32_t bvh_width, trace_block, trace_chunk, lds_stack, shade_waves, n_tried;
@@ -218,6 +229,8 @@ typedef struct yn_tune_result {
     int32_t batch_factor;     /* work items per batch over paths in flight; 2: ended
                                  paths' slots are regenerated from the remaining work */
     int32_t trace_overlap;    /* closest-hit and any-hit traces on two streams        */
+    int32_t graph;            /* bounce loop replayed from a HIP graph                */
+    int32_t pad;
     double  mrays_per_s;      /* the chosen configuration on the tuning view */
 } yn_tune_result;
 int  yn_autotune(yn_ctx* ctx, int32_t scene_id, const yn_cam* cam, int32_t width, int32_t height,

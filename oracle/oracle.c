@@ -1262,6 +1262,20 @@ void or_tonemap(const float* in, int n, float* out)
     }
 }
 
+/* tonemap.cl:49-58 (tonemapJohnHable, the variant tonemap.cl:29 leaves
+ * commented out), then the same gamma as tonemap.cl:33. */
+static float hable(float x)
+{
+    const float A = 0.15f, B = 0.50f, Cc = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f;
+    return ((x * (A * x + Cc * B) + D * E) / (x * (A * x + B) + D * F)) - E / F;
+}
+
+void or_tonemap_hable(const float* in, int n, float* out)
+{
+    const float g = 1 / 2.2f;
+    for (int i = 0; i < 4 * n; ++i) out[i] = ynf_powf(hable(in[i]), g);
+}
+
 /* ------------------------------------------------------------------ */
 /* exported pieces for closed-form tests                                */
 /* ------------------------------------------------------------------ */

@@ -102,6 +102,8 @@ void or_sample(const or_scene* s, const yn_cam* cam, int W, int H,
 
 /* tonemap.cl:14-47 (Reinhard, lum_white = 1, then gamma 1/2.2). */
 void or_tonemap(const float* in_rgba, int n_pixels, float* out_rgba);
+/* tonemap.cl:49-58 (John Hable's curve), then gamma 1/2.2. */
+void or_tonemap_hable(const float* in_rgba, int n_pixels, float* out_rgba);
 
 /* ---- BSDF / sampling pieces exported for closed-form tests ---- */
 /* hit: p[4], n[4], material; returns direction in dir[4] and pdf */

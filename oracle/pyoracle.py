@@ -96,6 +96,7 @@ def lib():
         L.or_sample.argtypes = [C.POINTER(Scene), C.POINTER(Cam), C.c_int, C.c_int, C.POINTER(RenderParams),
                                 C.c_uint32, C.c_int, C.c_int, C.c_void_p]
         L.or_tonemap.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.or_tonemap_hable.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.or_cosine_sample.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p,
                                        C.POINTER(C.c_float)]
         L.or_phong_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32),
@@ -231,8 +232,9 @@ def params(mode=0, **over) -> RenderParams:
     return p
 
 
-def tonemap(img: np.ndarray) -> np.ndarray:
+def tonemap(img: np.ndarray, op: int = 0) -> np.ndarray:
+    """tonemap.cl: op 0 Reinhard (as shipped), op 1 John Hable's curve."""
     img = np.ascontiguousarray(img, np.float32)
     out = np.empty_like(img)
-    lib().or_tonemap(_p(img), img.size // 4, _p(out))
+    (lib().or_tonemap_hable if op == 1 else lib().or_tonemap)(_p(img), img.size // 4, _p(out))
     return out

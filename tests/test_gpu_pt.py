@@ -302,3 +302,29 @@ def test_trace_overlap_keeps_images(mode):
             job.render(F)
             imgs.append(job.read())
     assert np.array_equal(imgs[0].view(np.int32), imgs[1].view(np.int32))
+
+
+@pytest.mark.parametrize("mode", [Y.YN_MODE_PT, Y.YN_MODE_BDPT])
+def test_graph_replay_keeps_images(mode):
+    """The bounce loop replayed from a captured HIP graph gives the same
+    images as launched kernel by kernel, bit for bit: over many replays of one
+    graph (frames rendered one call at a time, with host work and
+    synchronisation between them) and when the arguments change (two scenes
+    and sizes alternating, which captures again)."""
+    scenes = [Y.gen_icosphere_room(3, 9, 2), Y.gen_cornell()]
+    sizes = [(64, 48), (40, 40)]
+    kw = dict(bdpt_bounces=4) if mode else dict(mis=1, max_bounces=8, rr_threshold=2)
+    p = Y.params_reference(mode, **kw)
+    imgs = []
+    for gr in (0, 1):
+        with Y.Context(max_paths=1 << 14, graph=gr, lds_stack=8) as c:
+            assert c.config()["graph"] == gr
+            jobs = [c.job(c.scene(s), s.cam, W, H, p, base_seed=23 + k)
+                    for k, (s, (W, H)) in enumerate(zip(scenes, sizes))]
+            for _ in range(3):
+                for j in jobs:
+                    j.render(1)
+            jobs[0].render(2)
+            imgs.append([j.read() for j in jobs])
+    for a, b in zip(*imgs):
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))

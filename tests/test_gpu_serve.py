@@ -160,6 +160,27 @@ def test_save_hdr_png_flip_and_tonemap(tmp_path):
         job.save(str(tmp_path / "a.hdr"))
         job.save(str(tmp_path / "a.png"))
     assert np.array_equal(tm.view(np.int32), O.tonemap(img).view(np.int32))
+    # the Hable variant on the device equals the oracle's, bit for bit, both
+    # through the job (its .png) and on caller-owned device images
+    with Y.Context(max_paths=1 << 14) as c:
+        job = c.job(c.scene(s), s.cam, W, H, Y.params_reference(mis=1, max_bounces=8, rr_threshold=2), base_seed=3)
+        job.render(4)
+        job.set_tonemap(Y.TONEMAP_HABLE)
+        th = job.read_tonemapped()
+        job.save(str(tmp_path / "h.png"))
+        import torch
+        src = torch.from_numpy(img).cuda()
+        dst = torch.empty_like(src)
+        c.tonemap(dst, src, W, H, Y.TONEMAP_HABLE)
+        td = dst.cpu().numpy()
+        with pytest.raises(Y.YuneError):
+            job.set_tonemap(2)
+    oh = O.tonemap(img, 1)
+    assert np.array_equal(th.view(np.int32), oh.view(np.int32))
+    assert np.array_equal(td.view(np.int32), oh.view(np.int32))
+    hp = _read_png(tmp_path / "h.png")
+    exp = np.rint(np.clip(np.nan_to_num(oh[..., :3], nan=0.0), 0, 1) * 255).astype(np.uint8)[::-1]
+    assert np.abs(hp.astype(int) - exp.astype(int)).max() <= 1
     png = _read_png(tmp_path / "a.png")
     exp = np.rint(np.clip(np.nan_to_num(tm[..., :3], nan=0.0), 0, 1) * 255).astype(np.uint8)[::-1]
     assert np.abs(png.astype(int) - exp.astype(int)).max() <= 1    # lround vs rint on exact halves

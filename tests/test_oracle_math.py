@@ -385,6 +385,32 @@ def test_tonemap_is_gamma_with_unit_white():
     assert np.allclose(out, img.astype(np.float64) ** (1 / 2.2), rtol=3e-6)
 
 
+def test_tonemap_hable_closed_form():
+    """John Hable's curve (tonemap.cl:49-58), (x(Ax+CB)+DE)/(x(Ax+B)+DF) - E/F
+    with A..F = .15 .5 .1 .2 .02 .3, then gamma: zero at zero, monotonic,
+    tending to 1 - E/F = 14/15 for large x; values against float64."""
+    A, B, Cc, D, E, F = 0.15, 0.50, 0.10, 0.20, 0.02, 0.30
+    x = np.concatenate([[0.0, 0.5, 1.0, 11.2, 1e6], np.linspace(0.01, 50, 395)]).astype(np.float32)
+    img = np.stack([x, x, x, x], -1)
+    out = O.tonemap(img, 1)
+    xd = x.astype(np.float64)
+    curve = (xd * (A * xd + Cc * B) + D * E) / (xd * (A * xd + B) + D * F) - E / F
+    exp = np.sign(curve) * np.abs(curve) ** (1 / 2.2)
+    ok = curve > 1e-4
+    assert np.allclose(out[ok, 0], exp[ok], rtol=2e-5)
+    # at zero the float curve leaves one rounding residue, (DE)/(DF) - E/F =
+    # 2^-27, whose 1/2.2 power is ~2.0e-4
+    f = np.float32
+    z = (f(D) * f(E)) / (f(D) * f(F)) - f(E) / f(F)
+    assert z == f(2.0 ** -27) and abs(out[0, 0] - float(z) ** (1 / 2.2)) < 1e-8
+    assert (np.diff(out[5:, 0]) > 0).all()
+    assert abs(out[4, 0] - (14 / 15) ** (1 / 2.2)) < 1e-5
+    # the four channels go through the same curve
+    assert np.array_equal(out[:, 0].view(np.int32), out[:, 3].view(np.int32))
+    # and differ from the shipped Reinhard operator (the identity before gamma)
+    assert not np.allclose(out[1:4, 0], O.tonemap(img)[1:4, 0], rtol=1e-2)
+
+
 # ---------------------------------------------------------------- whole-path checks
 def box_scene(a, e, light_ke=(0, 0, 0)):
     """Closed cube [-1,1]^3 (12 triangles, normals inward), one grey

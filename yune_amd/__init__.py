@@ -21,11 +21,13 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import (MAT_DTYPE, NODE_DTYPE, TRI_DTYPE, YN_MODE_BDPT, YN_MODE_PT, Cam, CamCtl, Config, HostScene,
+from ._lib import (MAT_DTYPE, NODE_DTYPE, TONEMAP_HABLE, TONEMAP_REINHARD, TRI_DTYPE, YN_MODE_BDPT, YN_MODE_PT, Cam,
+                   CamCtl, Config, HostScene,
                    JobStatus, TuneResult,
                    LightQuad, RenderParams, SceneInfo, Stats, YuneError, check, lib, ptr)
 
-__all__ = ["TRI_DTYPE", "MAT_DTYPE", "NODE_DTYPE", "YN_MODE_PT", "YN_MODE_BDPT", "YuneError", "Scene",
+__all__ = ["TRI_DTYPE", "MAT_DTYPE", "NODE_DTYPE", "YN_MODE_PT", "YN_MODE_BDPT", "TONEMAP_REINHARD", "TONEMAP_HABLE",
+           "YuneError", "Scene",
            "build_bvh", "load_obj", "gen_cornell", "gen_icosphere_room", "camera", "params_reference",
            "Context", "Job", "light_quad"]
 
@@ -156,9 +158,9 @@ class Context:
     def __init__(self, device: int = 0, max_paths: int = 0, bvh_width: int = 0, trace_block: int = 0,
                  trace_chunk: int = 0, trace_grid: int = 0, mem_budget: int = 0, hoist_frac: float = 0.0,
                  leaf_max: int = 0, lds_stack: int = 0, shade_waves: int = 0, batch_factor: int = 0,
-                 trace_overlap: int = 0):
+                 trace_overlap: int = 0, graph: int = 0):
         cfg = Config(device, max_paths, mem_budget, bvh_width, trace_block, trace_chunk, trace_grid, hoist_frac,
-                     leaf_max, lds_stack, shade_waves, batch_factor, trace_overlap)
+                     leaf_max, lds_stack, shade_waves, batch_factor, trace_overlap, graph)
         self._p = C.c_void_p()
         check(lib().yn_create(C.byref(cfg), C.byref(self._p)))
         self.cfg = cfg
@@ -226,6 +228,17 @@ class Context:
                                      C.c_void_p(in_img.data_ptr()) if in_img is not None else None,
                                      C.byref(cam), sid, width, height, C.byref(params), gi_check, reset, rand,
                                      block, block_x, block_y))
+
+    def tonemap(self, out_img, in_img, width: int, height: int, op: int = 0):
+        """tonemap.cl on device images (H x W x 4 float32 CUDA tensors):
+        op TONEMAP_REINHARD (as shipped) or TONEMAP_HABLE."""
+        import torch
+        for t in (out_img, in_img):
+            if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.numel() == width * height * 4):
+                raise ValueError("images must be contiguous float32 CUDA tensors of height x width x 4")
+        torch.cuda.synchronize()
+        self._ck(lib().yn_tonemap_op(self._p, C.c_void_p(in_img.data_ptr()), C.c_void_p(out_img.data_ptr()),
+                                     width, height, op))
 
     def submit(self, sid: int, cam: Cam, width: int, height: int, params: RenderParams, frames: int,
                base_seed: int = 1) -> "Job":
@@ -300,6 +313,11 @@ class Job:
         img = np.empty((self.height, self.width, 4), np.float32)
         self.ctx._ck(lib().yn_job_read_tonemapped(self.ctx._p, self.id, ptr(img)))
         return img
+
+    def set_tonemap(self, op: int):
+        """TONEMAP_REINHARD (the shipped tonemap.cl) or TONEMAP_HABLE, for
+        read_tonemapped and .png saves."""
+        self.ctx._ck(lib().yn_job_set_tonemap(self.ctx._p, self.id, op))
 
     def save(self, path: str):
         self.ctx._ck(lib().yn_job_save(self.ctx._p, self.id, path.encode()))

@@ -50,7 +50,7 @@ class Config(C.Structure):
                 ("bvh_width", C.c_int32), ("trace_block", C.c_int32), ("trace_chunk", C.c_int32),
                 ("trace_grid", C.c_int32), ("hoist_frac", C.c_float), ("leaf_max", C.c_int32),
                 ("lds_stack", C.c_int32), ("shade_waves", C.c_int32), ("batch_factor", C.c_int32),
-                ("trace_overlap", C.c_int32)]
+                ("trace_overlap", C.c_int32), ("graph", C.c_int32), ("pad", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -77,7 +77,7 @@ class TuneResult(C.Structure):
     _fields_ = [("bvh_width", C.c_int32), ("trace_block", C.c_int32), ("trace_chunk", C.c_int32),
                 ("lds_stack", C.c_int32), ("shade_waves", C.c_int32), ("n_tried", C.c_int32),
                 ("batch_paths", C.c_int64), ("batch_factor", C.c_int32), ("trace_overlap", C.c_int32),
-                ("mrays_per_s", C.c_double)]
+                ("graph", C.c_int32), ("pad", C.c_int32), ("mrays_per_s", C.c_double)]
 
 
 class JobStatus(C.Structure):
@@ -96,6 +96,7 @@ assert C.sizeof(Cam) == 80 and C.sizeof(LightQuad) == 80
 assert HostScene.light.offset == 32 and HostScene.cam.offset == 112 and C.sizeof(HostScene) == 192
 
 YN_MODE_PT, YN_MODE_BDPT = 0, 1
+TONEMAP_REINHARD, TONEMAP_HABLE = 0, 1
 STATUS = {0: "YN_OK", -1: "YN_ERR_HIP", -2: "YN_ERR_INVALID", -3: "YN_ERR_NOT_FOUND", -4: "YN_ERR_NO_MEMORY",
           -5: "YN_ERR_IO", -6: "YN_ERR_NO_DEVICE"}
 
@@ -123,6 +124,8 @@ SIGNATURES = {
     "yn_pathtracer": (I32, [P, P, P, C.POINTER(Cam), I32, I32, I32, C.POINTER(RenderParams), I32, I32,
                             C.c_uint32, I32, I32, I32]),
     "yn_tonemap": (I32, [P, P, P, I32, I32]),
+    "yn_tonemap_op": (I32, [P, P, P, I32, I32, I32]),
+    "yn_job_set_tonemap": (I32, [P, I32, I32]),
     "yn_job_create": (I32, [P, I32, C.POINTER(Cam), I32, I32, C.POINTER(RenderParams), C.c_uint32, PI32]),
     "yn_job_render": (I32, [P, I32, I32]),
     "yn_job_read": (I32, [P, I32, P]),
@@ -165,6 +168,14 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C yune_amd/csrc)")
+        # torch ships its own libamdhip64.so.7 and its libraries name that copy.
+        # Loaded first, it also serves this library (same soname); loaded
+        # after ROCm's copy, it would be a second HIP runtime in the process
+        # and torch's device calls would fail.  One runtime for both.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

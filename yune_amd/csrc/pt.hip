@@ -466,18 +466,34 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a)
     a.accum[pix] = st4(acc);
 }
 
-// tonemap.cl:14-47: Reinhard with lum_white = 1 (identity up to rounding),
-// clamp discarded as in the reference, gamma 1/2.2.
-__global__ __launch_bounds__(256) void k_tonemap(const float4* in, float4* out, int n)
+// John Hable's filmic curve (tonemap.cl:49-58) on one channel: shoulder
+// 0.15, linear 0.50 at angle 0.10, toe 0.20 with 0.02 / 0.30; no white-point
+// division, as in the reference.
+__device__ __forceinline__ float hable_curve(float x)
+{
+    const float sA = 0.15f, sB = 0.50f, sC = 0.10f, sD = 0.20f, sE = 0.02f, sF = 0.30f;
+    return (x * (sA * x + sC * sB) + sD * sE) / (x * (sA * x + sB) + sD * sF) - sE / sF;
+}
+
+// tonemap.cl:14-47, op 0: Reinhard with lum_white = 1 (identity up to
+// rounding); op 1: the Hable curve the reference ships commented out
+// (tonemap.cl:29).  The clamp is discarded as in the reference, then gamma
+// 1/2.2.
+__global__ __launch_bounds__(256) void k_tonemap(const float4* in, float4* out, int n, int op)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     f4 col = ld(in[i]);
-    float lum_world = 0.212671f * col.x + 0.715160f * col.y + 0.072169f * col.z + 0.001f;
-    float lum_white = 1.0f;
-    float lum_display = lum_world * (1 + lum_world / (lum_white * lum_white)) / (1 + lum_world);
-    f4 sc = col / lum_world;
-    f4 ldr = mk(ynf_powf(sc.x, 1.0f), ynf_powf(sc.y, 1.0f), ynf_powf(sc.z, 1.0f), ynf_powf(sc.w, 1.0f)) * lum_display;
+    f4 ldr;
+    if (op == 1) {
+        ldr = mk(hable_curve(col.x), hable_curve(col.y), hable_curve(col.z), hable_curve(col.w));
+    } else {
+        float lum_world = 0.212671f * col.x + 0.715160f * col.y + 0.072169f * col.z + 0.001f;
+        float lum_white = 1.0f;
+        float lum_display = lum_world * (1 + lum_world / (lum_white * lum_white)) / (1 + lum_world);
+        f4 sc = col / lum_world;
+        ldr = mk(ynf_powf(sc.x, 1.0f), ynf_powf(sc.y, 1.0f), ynf_powf(sc.z, 1.0f), ynf_powf(sc.w, 1.0f)) * lum_display;
+    }
     const float g = 1 / 2.2f;
     out[i] = make_float4(ynf_powf(ldr.x, g), ynf_powf(ldr.y, g), ynf_powf(ldr.z, g), ynf_powf(ldr.w, g));
 }
@@ -530,9 +546,9 @@ void launch_resolve(const ResolveArgs& a, hipStream_t s)
     int n = a.w * a.h;
     if (n > 0) hipLaunchKernelGGL(k_resolve, dim3((n + 255) / 256), dim3(256), 0, s, a);
 }
-void launch_tonemap(const float4* in, float4* out, int n, hipStream_t s)
+void launch_tonemap(const float4* in, float4* out, int n, int op, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n);
+    if (n > 0) hipLaunchKernelGGL(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n, op);
 }
 void launch_prepare(const DevScene* scene, const float* rays, int n, RayQueue q, int32_t* light_out, hipStream_t s)
 {

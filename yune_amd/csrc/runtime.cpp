@@ -59,6 +59,7 @@ struct JobRec {
     float4* accum = nullptr;
     yn_stats st{};
     int counting = 0;
+    int tonemap_op = 0;           // 0: Reinhard (the shipped kernel), 1: Hable
     // pending capture (host pointers)
     int32_t* cap_prim_tri = nullptr;
     int32_t* cap_prim_light = nullptr;
@@ -89,6 +90,8 @@ struct yn_ctx {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;    // any-hit traces when they overlap the closest-hit ones
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    hipGraphExec_t gexec = nullptr;   // captured group of bounce iterations
+    std::vector<uint8_t> gkey;        // the launch arguments it was captured with
     std::string err;
     yn_config cfg{};
     std::vector<SceneRec> scenes;
@@ -414,98 +417,145 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     struct Span { hipEvent_t a, b; int kind; };
     std::vector<Span> spans;
 
+    // One bounce iteration's launches; every argument follows from the
+    // iteration's parity (queues ping-pong), so a group of 4 iterations is the
+    // same sequence of launches each time and may be replayed from a graph.
+    struct IterArgs { TraceArgs te, ts; ShadeArgs sa; BdptShadeArgs ba; };
+    auto make_iter = [&](int cur, IterArgs& A) {
+        std::memset(&A, 0, sizeof A);
+        const int nxt = cur ^ 1;
+        TraceArgs& ta = A.te;
+        ta.scenes = scenes;
+        ta.jobs = c->d_jobs;
+        ta.path_job = c->P.job;
+        ta.single_scene = single;
+        ta.chunk = c->cfg.trace_chunk;
+        ta.ctr = c->ctr;
+        ta.ovf_ref = c->ovf_ref;
+        ta.ovf_t = c->ovf_t;
+        ta.ovf_depth = c->ovf_depth;
+        // extension rays
+        ta.q = c->E[cur];
+        ta.hit_out = c->P.hit;
+        ta.work = q + 4;
+        ta.zero0 = c->E[nxt].count;
+        ta.zero1 = SQ[nxt].count;
+        // shadow rays
+        TraceArgs& ts = A.ts;
+        ts = ta;
+        ts.q = SQ[cur];
+        ts.hit_out = nullptr;
+        ts.occ_out = bdpt ? c->B.occ : c->P.occ;
+        ts.id_shift = bdpt ? c->B.shift : 0;
+        ts.work = q + 5;
+        ts.zero0 = bdpt ? q + 8 + nxt : nullptr;    // the next path list
+        ts.zero1 = nullptr;
+        if (overlap) {          // the second half of the overflow area
+            ts.ovf_ref = c->ovf_ref + (size_t)c->ovf_depth * grid * block;
+            ts.ovf_t = c->ovf_t + (size_t)c->ovf_depth * grid * block;
+        }
+        if (bdpt) {
+            BdptShadeArgs& ba = A.ba;
+            ba.scenes = scenes;
+            ba.jobs = c->d_jobs;
+            ba.P = c->P;
+            ba.B = c->B;
+            ba.list_in = c->lst[cur];
+            ba.list_in_count = q + 8 + cur;
+            ba.list_out = c->lst[nxt];
+            ba.list_out_count = q + 8 + nxt;
+            ba.ext_out = c->E[nxt];
+            ba.sh_out = c->ShB[nxt];
+            ba.zero0 = q + 4;
+            ba.zero1 = q + 5;
+        } else {
+            ShadeArgs& sa = A.sa;
+            sa.scenes = scenes;
+            sa.jobs = c->d_jobs;
+            sa.P = c->P;
+            sa.ext_in = c->E[cur];
+            sa.sh_in = c->Sh[cur];
+            sa.ext_out = c->E[nxt];
+            sa.sh_out = c->Sh[nxt];
+            sa.zero0 = q + 4;
+            sa.zero1 = q + 5;
+            sa.ctr = c->ctr;
+            sa.free_list = regen ? c->free_list : nullptr;
+            sa.free_count = q + 10;
+        }
+    };
+    if (max_depth > trace_lds_stack(c->cfg.lds_stack) + c->ovf_depth)
+        return fail(c, YN_ERR_INVALID, "BVH too deep for the traversal stack");
+    auto enqueue_iter = [&](const IterArgs& A, bool timed) -> int {
+        // the any-hit trace depends only on the previous shade, like the
+        // closest-hit one: with overlap it runs on the second stream
+        hipStream_t s2 = overlap ? c->stream2 : s;
+        if (overlap) {
+            HIPCK(c, hipEventRecord(c->fork_ev, s));
+            HIPCK(c, hipStreamWaitEvent(s2, c->fork_ev, 0));
+        }
+        hipEvent_t e0 = timed ? ev() : nullptr, e1 = timed ? ev() : nullptr;
+        if (timed) hipEventRecord(e0, s);
+        launch_trace(A.te, false, width, max_depth, block, grid, count, c->cfg.lds_stack, s);
+        if (timed) { hipEventRecord(e1, s); spans.push_back({e0, e1, 0}); }
+        hipEvent_t e2 = timed ? ev() : nullptr, e3 = timed ? ev() : nullptr;
+        if (timed) hipEventRecord(e2, s2);
+        launch_trace(A.ts, true, width, max_depth, block, grid, count, c->cfg.lds_stack, s2);
+        if (timed) { hipEventRecord(e3, s2); spans.push_back({e2, e3, 1}); }
+        if (overlap) {
+            HIPCK(c, hipEventRecord(c->join_ev, s2));
+            HIPCK(c, hipStreamWaitEvent(s, c->join_ev, 0));
+        }
+        hipEvent_t e4 = timed ? ev() : nullptr, e5 = timed ? ev() : nullptr;
+        if (timed) hipEventRecord(e4, s);
+        if (bdpt) launch_shade_bdpt(A.ba, cap, shade_grid, s);
+        else launch_shade_pt(A.sa, cap, shade_grid, c->cfg.shade_waves, s);
+        if (timed) { hipEventRecord(e5, s); spans.push_back({e4, e5, 2}); }
+        return 0;
+    };
+    IterArgs iters[2];
+    make_iter(0, iters[0]);
+    make_iter(1, iters[1]);
+    const int group = 4;                 // iterations between host checks (even)
+    // Graph replay of a group: the launches are captured once and replayed
+    // while their arguments stay byte-identical (same buffers, scenes, launch
+    // shapes). Not with regeneration (the free list alternates between two
+    // buffers) nor with ray capture for tests.
+    const bool any_cap = cap.prim_tri || cap.prim_light || cap.prim_t || cap.shadow_o || cap.shadow_d || cap.shadow_occ;
+    const bool use_graph = c->cfg.graph != 0 && !regen && !any_cap;
+    if (use_graph) {
+        std::vector<uint8_t> key(sizeof iters);
+        std::memcpy(key.data(), iters, sizeof iters);
+        const int32_t shape[] = {bdpt, width, max_depth, block, grid, shade_grid, count, c->cfg.lds_stack,
+                                 c->cfg.shade_waves, overlap, group};
+        key.insert(key.end(), (const uint8_t*)shape, (const uint8_t*)shape + sizeof shape);
+        if (!c->gexec || key != c->gkey) {
+            if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+            c->gkey.clear();
+            HIPCK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int r = 0;
+            for (int g = 0; g < group && !r; ++g) r = enqueue_iter(iters[g & 1], false);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            if (!r && e == hipSuccess) e = hipGraphInstantiateWithFlags(&c->gexec, graph, 0);
+            if (graph) hipGraphDestroy(graph);
+            if (r) return r;
+            if (e != hipSuccess) { c->gexec = nullptr; return fail(c, YN_ERR_HIP, std::string("bounce-loop graph: ") + hipGetErrorString(e)); }
+            c->gkey = std::move(key);
+        }
+    }
+
     auto t0 = std::chrono::steady_clock::now();
     int it = 0;
     for (;;) {
-        const int group = 4;
-        for (int g = 0; g < group; ++g, ++it) {
-            int cur = it & 1, nxt = cur ^ 1;
-            TraceArgs ta{};
-            ta.scenes = scenes;
-            ta.jobs = c->d_jobs;
-            ta.path_job = c->P.job;
-            ta.single_scene = single;
-            ta.chunk = c->cfg.trace_chunk;
-            ta.ctr = c->ctr;
-            ta.ovf_ref = c->ovf_ref;
-            ta.ovf_t = c->ovf_t;
-            ta.ovf_depth = c->ovf_depth;
-            // extension rays
-            ta.q = c->E[cur];
-            ta.hit_out = c->P.hit;
-            ta.occ_out = nullptr;
-            ta.work = q + 4;
-            ta.zero0 = c->E[nxt].count;
-            ta.zero1 = SQ[nxt].count;
-            bool timed = (int)spans.size() + 3 < kEvPool / 2;
-            // the any-hit trace depends only on the previous shade, like the
-            // closest-hit one: with overlap it runs on the second stream
-            hipStream_t s2 = overlap ? c->stream2 : s;
-            if (overlap) {
-                HIPCK(c, hipEventRecord(c->fork_ev, s));
-                HIPCK(c, hipStreamWaitEvent(s2, c->fork_ev, 0));
+        if (use_graph) {
+            HIPCK(c, hipGraphLaunch(c->gexec, s));
+            it += group;
+        } else {
+            for (int g = 0; g < group; ++g, ++it) {
+                bool timed = (int)spans.size() + 3 < kEvPool / 2;
+                if (int r = enqueue_iter(iters[it & 1], timed)) return r;
             }
-            hipEvent_t e0 = timed ? ev() : nullptr, e1 = timed ? ev() : nullptr;
-            if (timed) hipEventRecord(e0, s);
-            if (launch_trace(ta, false, width, max_depth, block, grid, count, c->cfg.lds_stack, s) != 0)
-                return fail(c, YN_ERR_INVALID, "BVH too deep for the traversal stack");
-            if (timed) { hipEventRecord(e1, s); spans.push_back({e0, e1, 0}); }
-            // shadow rays
-            ta.q = SQ[cur];
-            ta.hit_out = nullptr;
-            ta.occ_out = bdpt ? c->B.occ : c->P.occ;
-            ta.id_shift = bdpt ? c->B.shift : 0;
-            ta.work = q + 5;
-            ta.zero0 = bdpt ? q + 8 + nxt : nullptr;    // the next path list
-            ta.zero1 = nullptr;
-            if (overlap) {          // the second half of the overflow area
-                ta.ovf_ref = c->ovf_ref + (size_t)c->ovf_depth * grid * block;
-                ta.ovf_t = c->ovf_t + (size_t)c->ovf_depth * grid * block;
-            }
-            hipEvent_t e2 = timed ? ev() : nullptr, e3 = timed ? ev() : nullptr;
-            if (timed) hipEventRecord(e2, s2);
-            launch_trace(ta, true, width, max_depth, block, grid, count, c->cfg.lds_stack, s2);
-            ta.id_shift = 0;
-            if (timed) { hipEventRecord(e3, s2); spans.push_back({e2, e3, 1}); }
-            if (overlap) {
-                HIPCK(c, hipEventRecord(c->join_ev, s2));
-                HIPCK(c, hipStreamWaitEvent(s, c->join_ev, 0));
-            }
-            // shade
-            hipEvent_t e4 = timed ? ev() : nullptr, e5 = timed ? ev() : nullptr;
-            if (timed) hipEventRecord(e4, s);
-            if (bdpt) {
-                BdptShadeArgs ba{};
-                ba.scenes = scenes;
-                ba.jobs = c->d_jobs;
-                ba.P = c->P;
-                ba.B = c->B;
-                ba.list_in = c->lst[cur];
-                ba.list_in_count = q + 8 + cur;
-                ba.list_out = c->lst[nxt];
-                ba.list_out_count = q + 8 + nxt;
-                ba.ext_out = c->E[nxt];
-                ba.sh_out = c->ShB[nxt];
-                ba.zero0 = q + 4;
-                ba.zero1 = q + 5;
-                launch_shade_bdpt(ba, cap, shade_grid, s);
-            } else {
-                ShadeArgs sa{};
-                sa.scenes = scenes;
-                sa.jobs = c->d_jobs;
-                sa.P = c->P;
-                sa.ext_in = c->E[cur];
-                sa.sh_in = c->Sh[cur];
-                sa.ext_out = c->E[nxt];
-                sa.sh_out = c->Sh[nxt];
-                sa.zero0 = q + 4;
-                sa.zero1 = q + 5;
-                sa.ctr = c->ctr;
-                sa.free_list = regen ? c->free_list : nullptr;
-                sa.free_count = q + 10;
-                launch_shade_pt(sa, cap, shade_grid, c->cfg.shade_waves, s);
-            }
-            if (timed) { hipEventRecord(e5, s); spans.push_back({e4, e5, 2}); }
         }
         HIPCK(c, hipGetLastError());
         HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, s));
@@ -883,6 +933,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.batch_factor == 0) k.batch_factor = 1;
     if (k.batch_factor != 1 && k.batch_factor != 2) return fail(nullptr, YN_ERR_INVALID, "batch_factor must be 1 or 2");
     if (k.trace_overlap != 0 && k.trace_overlap != 1) return fail(nullptr, YN_ERR_INVALID, "trace_overlap must be 0 or 1");
+    if (k.graph != 0 && k.graph != 1) return fail(nullptr, YN_ERR_INVALID, "graph must be 0 or 1");
     if (k.bvh_width != 2 && k.bvh_width != 4) return fail(nullptr, YN_ERR_INVALID, "bvh_width must be 2 or 4");
     if (k.trace_block != 64 && k.trace_block != 128 && k.trace_block != 256)
         return fail(nullptr, YN_ERR_INVALID, "trace_block must be 64, 128 or 256");
@@ -932,6 +983,7 @@ void yn_destroy(yn_ctx* c)
 {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
     free_all(c);
     if (c->d_segs) hipFree(c->d_segs);
     if (c->ovf_ref) hipFree(c->ovf_ref);
@@ -1123,7 +1175,7 @@ int yn_job_read_tonemapped(yn_ctx* c, int32_t jid, float* rgba)
     float4* tmp = nullptr;
     int r = dalloc(c, &tmp, n);
     if (r) return r;
-    launch_tonemap(J.accum, tmp, (int)n, c->stream);
+    launch_tonemap(J.accum, tmp, (int)n, J.tonemap_op, c->stream);
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipMemcpy(rgba, tmp, n * 16, hipMemcpyDeviceToHost);
     hipFree(tmp);
@@ -1310,6 +1362,7 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         {5, {cap, cap / 2, cap / 4}},           // paths per batch
         {6, {1, 2}},                            // work per batch over paths in flight (regeneration)
         {7, {0, 1}},                            // overlapping closest-hit and any-hit traces
+        {8, {0, 1}},                            // bounce loop replayed from a graph
     };
     auto get = [&](int kind) -> int64_t {
         switch (kind) {
@@ -1320,7 +1373,8 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         case 4: return c->cfg.shade_waves;
         case 5: return c->batch_paths > 0 ? c->batch_paths : cap;
         case 6: return c->cfg.batch_factor;
-        default: return c->cfg.trace_overlap;
+        case 7: return c->cfg.trace_overlap;
+        default: return c->cfg.graph;
         }
     };
     auto set = [&](int kind, int64_t v) -> int {
@@ -1332,7 +1386,8 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         case 4: c->cfg.shade_waves = (int)v; return 0;
         case 5: c->batch_paths = v; return 0;
         case 6: c->cfg.batch_factor = (int)v; return 0;
-        default: c->cfg.trace_overlap = (int)v; return 0;
+        case 7: c->cfg.trace_overlap = (int)v; return 0;
+        default: c->cfg.graph = (int)v; return 0;
         }
     };
     double best = measure();
@@ -1361,6 +1416,7 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
     out->batch_paths = c->batch_paths > 0 ? c->batch_paths : cap;
     out->batch_factor = c->cfg.batch_factor;
     out->trace_overlap = c->cfg.trace_overlap;
+    out->graph = c->cfg.graph;
     out->mrays_per_s = best / 1e6;
     out->n_tried = tried;
     return 0;
@@ -1419,12 +1475,26 @@ int yn_pathtracer(yn_ctx* c, float* out_img, const float* in_img, const yn_cam* 
     return r;
 }
 
-int yn_tonemap(yn_ctx* c, const float* in_img, float* out_img, int32_t W, int32_t H)
+int yn_tonemap_op(yn_ctx* c, const float* in_img, float* out_img, int32_t W, int32_t H, int32_t op)
 {
     if (check_ctx(c)) return YN_ERR_INVALID;
     if (!in_img || !out_img || W <= 0 || H <= 0) return fail(c, YN_ERR_INVALID, "bad tonemap arguments");
-    launch_tonemap((const float4*)in_img, (float4*)out_img, W * H, c->stream);
+    if (op != YN_TONEMAP_REINHARD && op != YN_TONEMAP_HABLE) return fail(c, YN_ERR_INVALID, "unknown tonemap operator");
+    launch_tonemap((const float4*)in_img, (float4*)out_img, W * H, op, c->stream);
     HIPCK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int yn_tonemap(yn_ctx* c, const float* in_img, float* out_img, int32_t W, int32_t H)
+{
+    return yn_tonemap_op(c, in_img, out_img, W, H, YN_TONEMAP_REINHARD);
+}
+
+int yn_job_set_tonemap(yn_ctx* c, int32_t jid, int32_t op)
+{
+    if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
+    if (op != YN_TONEMAP_REINHARD && op != YN_TONEMAP_HABLE) return fail(c, YN_ERR_INVALID, "unknown tonemap operator");
+    c->jobs[jid].tonemap_op = op;
     return 0;
 }
 

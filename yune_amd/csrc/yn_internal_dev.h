@@ -154,7 +154,7 @@ void launch_generate_bdpt(const GenArgs& a, const DevCapture& cap, hipStream_t s
 void launch_shade_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int grid, hipStream_t stream);
 void launch_resolve(const ResolveArgs& a, hipStream_t stream);
 void launch_fmath(int op, const float* x, const float* y, int n, float* out, hipStream_t stream);
-void launch_tonemap(const float4* in, float4* out, int n, hipStream_t stream);
+void launch_tonemap(const float4* in, float4* out, int n, int op, hipStream_t stream);
 void launch_prepare(const DevScene* scene, const float* rays, int n, RayQueue q, int32_t* light_out,
                     hipStream_t stream);
 
