@@ -363,7 +363,8 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     uint32_t* q = c->ctr->q;
     // counters: q[0],q[1] extension queue counts, q[2],q[3] shadow queue counts,
     // q[4],q[5] work counters of the extend / shadow launches, q[7] stack
-    // overflow flag, q[8],q[9] path-list counts (bidirectional mode).
+    // overflow flag, q[8],q[9] path-list counts (bidirectional mode), q[10]
+    // free slots (regeneration).
     for (int k = 0; k < 2; ++k) {
         c->E[k].count = q + k;
         c->Sh[k].count = q + 2 + k;
@@ -577,6 +578,8 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
                 HIPCK(c, hipcub::DeviceRadixSort::SortKeys(c->sort_tmp, tb, c->free_list, c->free_alt, (int)f, 0,
                                                            c->slot_bits, s));
                 std::swap(c->free_list, c->free_alt);
+                make_iter(0, iters[0]);      // the shade appends to the other buffer now
+                make_iter(1, iters[1]);
                 GenArgs gr = ga;
                 gr.item0 = (int32_t)issued;
                 gr.n_paths = (int32_t)std::min<int64_t>((int64_t)f, need);
@@ -925,7 +928,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.hoist_frac == 0.0f) k.hoist_frac = 0.01f;
     if (k.leaf_max <= 0) k.leaf_max = 4;
     if (k.leaf_max > 10) k.leaf_max = 10;
-    if (k.lds_stack == 0) k.lds_stack = 16;
+    if (k.lds_stack == 0) k.lds_stack = 8;
     if (k.shade_waves == 0) k.shade_waves = 2;
     if (k.lds_stack != 8 && k.lds_stack != 16 && k.lds_stack != 32)
         return fail(nullptr, YN_ERR_INVALID, "lds_stack must be 8, 16 or 32");
@@ -1357,7 +1360,7 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         {0, {4, 2}},                            // BVH width
         {1, {256, 128, 64}},                    // trace block
         {2, {64, 32, 128}},                     // rays per wave per queue fetch
-        {3, {16, 8, 32}},                       // LDS stack entries
+        {3, {8, 16, 32}},                       // LDS stack entries
         {4, {2, 3, 4}},                         // shade occupancy
         {5, {cap, cap / 2, cap / 4}},           // paths per batch
         {6, {1, 2}},                            // work per batch over paths in flight (regeneration)

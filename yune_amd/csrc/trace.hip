@@ -275,8 +275,16 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
 
 }  // namespace
 
+// Occupancy the kernel is compiled for, per SIMD: the LDS stack (6 B per
+// entry and lane, 160 KB per CU) allows 13 / 6 / 3 waves per SIMD for 8 /
+// 16 / 32 entries, and registers are held to what 7 / 6 / 3 waves leave
+// (72 / 80 / 168).  More waves in flight hide more of the node-fetch
+// latency; past 7 the register spills cost more than they hide.
+template <int LST>
+constexpr int trace_waves() { return LST == 8 ? 7 : (LST == 16 ? 6 : 3); }
+
 template <int W, bool ANY, int LST, int BLOCK, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a)
+__global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a)
 {
     __shared__ uint32_t stk_all[LST * BLOCK];
     __shared__ uint16_t stkt_all[LST * BLOCK];
