@@ -57,7 +57,7 @@ typedef struct yn_config {
     int32_t lds_stack;        /* traversal stack entries per lane kept in LDS:
                                  8, 16 or 32 (0: 8); deeper entries spill to HBM   */
     int32_t shade_waves;      /* occupancy the shade kernel is compiled for: 2, 3, 4
-                                 (0: 2)                                             */
+                                 (0: 4)                                             */
     int32_t batch_factor;     /* path-tracing work per batch over paths in flight:
                                  1 (0: default) or 2, when ended paths' slots are
                                  regenerated from the remaining work               */

@@ -209,135 +209,139 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
 
     Out out;
     bool ended = false;
-    for (int guard = 0; guard < 16 && !ended; ++guard) {
-        if (step == S_NEE_START) {   // evaluateDirectLighting (udpt.cl:535-555)
-            Mat m = load_mat(S.mats, S.tri_mat[tri]);
-            em = m.ke;
-            f4 wraw;
-            if (!sample_lights(L, hp, hn, seed, lpdf, wraw) || lpdf <= 0.0f) { nee_res = em; step = S_NEE_DONE; continue; }
-            float len = length(wraw);
-            len -= kEps * 1.5f;
-            wi = normalize(wraw);
-            bool glossy = false;
-            nprob = 0.0f;
-            if (fix) {
-                glossy = sample_glossy_pdf(m, seed, nprob, false);
-                if (nprob == 0.0f) { nee_res = em; step = S_NEE_DONE; continue; }
-                st = glossy ? (st | F_NGLOSSY) : (st & ~F_NGLOSSY);
-            }
-            f4 so = hp + wi * kEps;
-            float tq;
-            if (cap_on && (st & F_DIRECT) && !(st & F_CAPSH)) {
-                cap.shadow_o[pix] = make_float4(so.x, so.y, so.z, len);
-                cap.shadow_d[pix] = make_float4(wi.x, wi.y, wi.z, 1.0f);
-            }
-            if (quad_hit(L, so, wi, len, tq)) { occ = 1; step = S_NEE_SHADOW; continue; }
-            P.g3[r4] = make_float4(wi.x, wi.y, wi.z, lpdf);
-            nee_prob = nprob;
-            np_dirty = true;
-            st = (st & ~ST_MASK) | ST_NEE_SH;
-            out = make_ray(2, so, wi, len, p);
-            break;
-        }
-        if (step == S_NEE_SHADOW) {  // udpt.cl:557-565 and the MIS light weight
-            if (code == ST_NEE_SH) {
-                float4 w4 = P.g3[r4];
-                wi = mk(w4.x, w4.y, w4.z, 0.0f);
-                lpdf = w4.w;
-                nprob = nee_prob;
-                code = ST_DONE;     // consumed
-            }
-            if (cap_on && (st & F_DIRECT) && !(st & F_CAPSH)) { cap.shadow_occ[pix] = occ; st |= F_CAPSH; }
-            Mat m = load_mat(S.mats, S.tri_mat[tri]);
-            em = m.ke;
-            f4 w_o = -rd;
-            bool glossy = fix ? ((st & F_NGLOSSY) != 0) : false;
-            ls = mk(0, 0, 0, 0);
-            if (!occ) {
-                if (!fix) {
-                    glossy = sample_glossy_pdf(m, seed, nprob, false);
-                    if (nprob == 0.0f) { nee_res = em; step = S_NEE_DONE; continue; }
-                }
-                ls = (eval_brdf(m, wi, w_o, hp, hn, glossy, nprob, on, false) * ld(L.ke)) * fmaxf(dot(wi, hn), 0.0f);
-                ls = ls * (1 / lpdf);
-            }
-            if (!RP.mis) { nee_res = ls + em; step = S_NEE_DONE; continue; }
-            float bpdf = glossy ? phong_pdf(m, wi, w_o, hn) : cos_pdf(wi, hn);
-            float mw = power_heuristic(lpdf, lpdf, bpdf);
-            ls = ls * mw;
-            f4 bd = glossy ? phong_sample(m, bpdf, w_o, hp, hn, seed, false) : cosine_sample(bpdf, hp, hn, seed);
-            if (bpdf <= 0.0f) { nee_res = ls + em; step = S_NEE_DONE; continue; }
-            f4 bo = hp + bd * kEps;
-            float tq;
-            if (!quad_hit(L, bo, bd, INFINITY, tq)) { nee_res = ls + em; step = S_NEE_DONE; continue; }
+    // The steps only ever move forward (NEE start, shadow, MIS, done, bounce
+    // sample, finish), so they run as one forward sequence: a step that
+    // hands over sets `step` to a later one and leaves its block, a step that
+    // makes a ray jumps past the rest.  Without a loop back-edge, values of
+    // one step are not kept live through the others.
+    if (step == S_NEE_START) do {   // evaluateDirectLighting (udpt.cl:535-555)
+        Mat m = load_mat(S.mats, S.tri_mat[tri]);
+        em = m.ke;
+        f4 wraw;
+        if (!sample_lights(L, hp, hn, seed, lpdf, wraw) || lpdf <= 0.0f) { nee_res = em; step = S_NEE_DONE; break; }
+        float len = length(wraw);
+        len -= kEps * 1.5f;
+        wi = normalize(wraw);
+        bool glossy = false;
+        nprob = 0.0f;
+        if (fix) {
+            glossy = sample_glossy_pdf(m, seed, nprob, false);
+            if (nprob == 0.0f) { nee_res = em; step = S_NEE_DONE; break; }
             st = glossy ? (st | F_NGLOSSY) : (st & ~F_NGLOSSY);
-            P.g3[r4 + 1] = st4(ls);
-            P.g3[r4 + 2] = make_float4(bd.x, bd.y, bd.z, bpdf);
-            P.g3[r4] = make_float4(wi.x, wi.y, wi.z, lpdf);
-            nee_prob = nprob;
-            np_dirty = true;
-            st = (st & ~ST_MASK) | ST_NEE_MIS;
-            out = make_ray(2, bo, bd, tq, p);
-            break;
         }
-        if (step == S_NEE_MIS) {     // udpt.cl:599-607
-            Mat m = load_mat(S.mats, S.tri_mat[tri]);
-            em = m.ke;
-            ls = ld(P.g3[r4 + 1]);
-            if (occ) { nee_res = ls + em; step = S_NEE_DONE; continue; }
-            float4 mb = P.g3[r4 + 2];
-            f4 bd = mk(mb.x, mb.y, mb.z, 0.0f);
-            float bpdf = mb.w;
-            lpdf = P.g3[r4].w;
+        f4 so = hp + wi * kEps;
+        float tq;
+        if (cap_on && (st & F_DIRECT) && !(st & F_CAPSH)) {
+            cap.shadow_o[pix] = make_float4(so.x, so.y, so.z, len);
+            cap.shadow_d[pix] = make_float4(wi.x, wi.y, wi.z, 1.0f);
+        }
+        if (quad_hit(L, so, wi, len, tq)) { occ = 1; step = S_NEE_SHADOW; break; }
+        P.g3[r4] = make_float4(wi.x, wi.y, wi.z, lpdf);
+        nee_prob = nprob;
+        np_dirty = true;
+        st = (st & ~ST_MASK) | ST_NEE_SH;
+        out = make_ray(2, so, wi, len, p);
+        goto steps_done;
+    } while (0);
+    if (step == S_NEE_SHADOW) do {  // udpt.cl:557-565 and the MIS light weight
+        if (code == ST_NEE_SH) {
+            float4 w4 = P.g3[r4];
+            wi = mk(w4.x, w4.y, w4.z, 0.0f);
+            lpdf = w4.w;
             nprob = nee_prob;
-            bool glossy = (st & F_NGLOSSY) != 0;
-            float mw = power_heuristic(bpdf, lpdf, bpdf);
-            f4 bs = (eval_brdf(m, bd, -rd, hp, hn, glossy, nprob, on, false) * ld(L.ke)) * fmaxf(dot(bd, hn), 0.0f);
-            bs = bs * (mw / bpdf);
-            nee_res = (ls + bs) + em;
-            step = S_NEE_DONE;
-            continue;
+            code = ST_DONE;     // consumed
         }
-        if (step == S_NEE_DONE) {
-            load_acc();
-            if (st & F_DIRECT) { direct = nee_res; dir_dirty = true; }
-            else { indirect = indirect + tp * nee_res; ind_dirty = true; }
-            step = S_BOUNCE_SAMPLE;
-            continue;
-        }
-        if (step == S_BOUNCE_SAMPLE) {   // udpt.cl:458-487
-            if (!RP.gi || bounce >= RP.max_bounces) { step = S_FINISH; continue; }
-            Mat m = load_mat(S.mats, S.tri_mat[tri]);
-            float pdf = 1.0f, bprob = 0.0f, ior = 1.0f;
-            bool glossy = false;
-            f4 nd;
-            if (m.spec) nd = fresnel_sample(m, hn, -rd, ior, seed);
-            else {
-                glossy = sample_glossy_pdf(m, seed, bprob, false);
-                if (bprob == 0.0f) { step = S_FINISH; continue; }
-                nd = glossy ? phong_sample(m, pdf, -rd, hp, hn, seed, false) : cosine_sample(pdf, hp, hn, seed);
+        if (cap_on && (st & F_DIRECT) && !(st & F_CAPSH)) { cap.shadow_occ[pix] = occ; st |= F_CAPSH; }
+        Mat m = load_mat(S.mats, S.tri_mat[tri]);
+        em = m.ke;
+        f4 w_o = -rd;
+        bool glossy = fix ? ((st & F_NGLOSSY) != 0) : false;
+        ls = mk(0, 0, 0, 0);
+        if (!occ) {
+            if (!fix) {
+                glossy = sample_glossy_pdf(m, seed, nprob, false);
+                if (nprob == 0.0f) { nee_res = em; step = S_NEE_DONE; break; }
             }
-            if (pdf <= 0.0f) { step = S_FINISH; continue; }
-            f4 no = hp + nd * kEps;
-            float tq, tmax = INFINITY;
-            st &= ~F_LCAND;
-            if (quad_hit(L, no, nd, INFINITY, tq)) { tmax = tq; st |= F_LCAND; }
-            st = glossy ? (st | F_BGLOSSY) : (st & ~F_BGLOSSY);
-            P.g2[r4 + 2] = st4(nd);
-            P.g2[r4 + 3] = make_float4(pdf, bprob, ior, 0.0f);
-            st = (st & ~ST_MASK) | ST_BOUNCE;
-            out = make_ray(1, no, nd, tmax, p);
-            break;
+            ls = (eval_brdf(m, wi, w_o, hp, hn, glossy, nprob, on, false) * ld(L.ke)) * fmaxf(dot(wi, hn), 0.0f);
+            ls = ls * (1 / lpdf);
         }
-        if (step == S_FINISH) {
-            load_acc();
-            write_sample(J, slot, pix, direct + indirect);
-            st = ST_DONE;
-            ended = true;
-            break;
+        if (!RP.mis) { nee_res = ls + em; step = S_NEE_DONE; break; }
+        float bpdf = glossy ? phong_pdf(m, wi, w_o, hn) : cos_pdf(wi, hn);
+        float mw = power_heuristic(lpdf, lpdf, bpdf);
+        ls = ls * mw;
+        f4 bd = glossy ? phong_sample(m, bpdf, w_o, hp, hn, seed, false) : cosine_sample(bpdf, hp, hn, seed);
+        if (bpdf <= 0.0f) { nee_res = ls + em; step = S_NEE_DONE; break; }
+        f4 bo = hp + bd * kEps;
+        float tq;
+        if (!quad_hit(L, bo, bd, INFINITY, tq)) { nee_res = ls + em; step = S_NEE_DONE; break; }
+        st = glossy ? (st | F_NGLOSSY) : (st & ~F_NGLOSSY);
+        P.g3[r4 + 1] = st4(ls);
+        P.g3[r4 + 2] = make_float4(bd.x, bd.y, bd.z, bpdf);
+        P.g3[r4] = make_float4(wi.x, wi.y, wi.z, lpdf);
+        nee_prob = nprob;
+        np_dirty = true;
+        st = (st & ~ST_MASK) | ST_NEE_MIS;
+        out = make_ray(2, bo, bd, tq, p);
+        goto steps_done;
+    } while (0);
+    if (step == S_NEE_MIS) do {     // udpt.cl:599-607
+        Mat m = load_mat(S.mats, S.tri_mat[tri]);
+        em = m.ke;
+        ls = ld(P.g3[r4 + 1]);
+        if (occ) { nee_res = ls + em; step = S_NEE_DONE; break; }
+        float4 mb = P.g3[r4 + 2];
+        f4 bd = mk(mb.x, mb.y, mb.z, 0.0f);
+        float bpdf = mb.w;
+        lpdf = P.g3[r4].w;
+        nprob = nee_prob;
+        bool glossy = (st & F_NGLOSSY) != 0;
+        float mw = power_heuristic(bpdf, lpdf, bpdf);
+        f4 bs = (eval_brdf(m, bd, -rd, hp, hn, glossy, nprob, on, false) * ld(L.ke)) * fmaxf(dot(bd, hn), 0.0f);
+        bs = bs * (mw / bpdf);
+        nee_res = (ls + bs) + em;
+        step = S_NEE_DONE;
+        break;
+    } while (0);
+    if (step == S_NEE_DONE) do {
+        load_acc();
+        if (st & F_DIRECT) { direct = nee_res; dir_dirty = true; }
+        else { indirect = indirect + tp * nee_res; ind_dirty = true; }
+        step = S_BOUNCE_SAMPLE;
+        break;
+    } while (0);
+    if (step == S_BOUNCE_SAMPLE) do {   // udpt.cl:458-487
+        if (!RP.gi || bounce >= RP.max_bounces) { step = S_FINISH; break; }
+        Mat m = load_mat(S.mats, S.tri_mat[tri]);
+        float pdf = 1.0f, bprob = 0.0f, ior = 1.0f;
+        bool glossy = false;
+        f4 nd;
+        if (m.spec) nd = fresnel_sample(m, hn, -rd, ior, seed);
+        else {
+            glossy = sample_glossy_pdf(m, seed, bprob, false);
+            if (bprob == 0.0f) { step = S_FINISH; break; }
+            nd = glossy ? phong_sample(m, pdf, -rd, hp, hn, seed, false) : cosine_sample(pdf, hp, hn, seed);
         }
-        if (step == S_FINISH_CONST) { write_sample(J, slot, pix, const_color); st = ST_DONE; ended = true; break; }
-    }
+        if (pdf <= 0.0f) { step = S_FINISH; break; }
+        f4 no = hp + nd * kEps;
+        float tq, tmax = INFINITY;
+        st &= ~F_LCAND;
+        if (quad_hit(L, no, nd, INFINITY, tq)) { tmax = tq; st |= F_LCAND; }
+        st = glossy ? (st | F_BGLOSSY) : (st & ~F_BGLOSSY);
+        P.g2[r4 + 2] = st4(nd);
+        P.g2[r4 + 3] = make_float4(pdf, bprob, ior, 0.0f);
+        st = (st & ~ST_MASK) | ST_BOUNCE;
+        out = make_ray(1, no, nd, tmax, p);
+        goto steps_done;
+    } while (0);
+    if (step == S_FINISH) do {
+        load_acc();
+        write_sample(J, slot, pix, direct + indirect);
+        st = ST_DONE;
+        ended = true;
+        goto steps_done;
+    } while (0);
+    if (step == S_FINISH_CONST) { write_sample(J, slot, pix, const_color); st = ST_DONE; ended = true; }
+steps_done:
     if (out.kind == 0 && !ended) {   // unreachable by construction; never leave a path hanging
         write_sample(J, slot, pix, mk(NAN, NAN, NAN, NAN));
         st = ST_DONE;
@@ -362,8 +366,9 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
 
 }  // namespace
 
-// WAVES: the occupancy the register allocation is held to (2: no limit on
-// this kernel's ~230 VGPRs; 3 and 4 trade spills for latency hiding).
+// WAVES: the occupancy the register allocation is held to, per SIMD (the
+// kernel needs ~133 VGPRs unconstrained, 3 waves; 4 fits in 128 without
+// spills; 5 would spill).
 template <int WAVES>
 __global__ __launch_bounds__(256, WAVES) void k_shade_pt(ShadeArgs a, DevCapture cap)
 {

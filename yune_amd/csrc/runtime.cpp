@@ -405,7 +405,10 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     }
     int block = c->cfg.trace_block;
     int grid = c->cfg.trace_grid > 0 ? c->cfg.trace_grid : c->n_cu * (2048 / block);
-    int shade_grid = c->n_cu * 8;
+    // persistent shade grids: one block of 4 waves per CU for each wave per
+    // SIMD the path-tracing shade is compiled for; the bidirectional shade
+    // keeps 8 blocks per CU
+    int shade_grid = bdpt ? c->n_cu * 8 : c->n_cu * c->cfg.shade_waves;
     const bool overlap = c->cfg.trace_overlap != 0;
     // overlapping traces get disjoint halves of the stack overflow area
     if (int r = ensure_overflow(c, max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * block * (overlap ? 2 : 1)))
@@ -929,7 +932,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.leaf_max <= 0) k.leaf_max = 4;
     if (k.leaf_max > 10) k.leaf_max = 10;
     if (k.lds_stack == 0) k.lds_stack = 8;
-    if (k.shade_waves == 0) k.shade_waves = 2;
+    if (k.shade_waves == 0) k.shade_waves = 4;
     if (k.lds_stack != 8 && k.lds_stack != 16 && k.lds_stack != 32)
         return fail(nullptr, YN_ERR_INVALID, "lds_stack must be 8, 16 or 32");
     if (k.shade_waves < 2 || k.shade_waves > 4) return fail(nullptr, YN_ERR_INVALID, "shade_waves must be 2, 3 or 4");
