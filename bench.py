@@ -263,6 +263,8 @@ def main():
     ap.add_argument("--batch-factor", type=int, default=0, help="work per batch over paths in flight (2: regeneration)")
     ap.add_argument("--trace-overlap", type=int, default=0, help="1: closest-hit and any-hit traces on two streams")
     ap.add_argument("--graph", type=int, default=0, help="1: bounce loop replayed from a HIP graph")
+    ap.add_argument("--leaf-max", type=int, default=0, help="triangles per BVH leaf (0: library default)")
+    ap.add_argument("--hoist-frac", type=float, default=0.0, help="box-area fraction above which triangles are hoisted")
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC counters")
     args = ap.parse_args()
     if args.copy_probe:
@@ -292,6 +294,7 @@ def main():
 
     params = Y.params_reference(wl["mode"], **wl["params"])
     with Y.Context(bvh_width=args.bvh_width, trace_block=args.trace_block, trace_chunk=args.trace_chunk,
+                   leaf_max=args.leaf_max, hoist_frac=args.hoist_frac,
                    trace_grid=args.trace_grid, lds_stack=args.lds_stack, shade_waves=args.shade_waves,
                    max_paths=args.max_paths, batch_factor=args.batch_factor,
                    trace_overlap=args.trace_overlap, graph=args.graph) as ctx:

@@ -53,7 +53,7 @@ typedef struct yn_config {
     float   hoist_frac;       /* triangles whose box area exceeds this fraction of the
                                  root box's are tested by every ray instead of living
                                  in the tree (0: 0.01; < 0: off)                    */
-    int32_t leaf_max;         /* leaves of more triangles are split (1..10; 0: 4)   */
+    int32_t leaf_max;         /* leaves of more triangles are split (1..10; 0: 2)   */
     int32_t lds_stack;        /* traversal stack entries per lane kept in LDS:
                                  8, 16 or 32 (0: 8); deeper entries spill to HBM   */
     int32_t shade_waves;      /* occupancy the shade kernel is compiled for: 2, 3, 4

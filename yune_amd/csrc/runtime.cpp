@@ -929,7 +929,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.trace_block == 0) k.trace_block = 256;
     if (k.trace_chunk <= 0) k.trace_chunk = 64;
     if (k.hoist_frac == 0.0f) k.hoist_frac = 0.01f;
-    if (k.leaf_max <= 0) k.leaf_max = 4;
+    if (k.leaf_max <= 0) k.leaf_max = 2;
     if (k.leaf_max > 10) k.leaf_max = 10;
     if (k.lds_stack == 0) k.lds_stack = 8;
     if (k.shade_waves == 0) k.shade_waves = 4;
