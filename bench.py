@@ -100,7 +100,7 @@ def serve_bench(args, Y, np):
     n_jobs = args.jobs or (int(args.duration * args.rate) if serve else 64)
     jobs = serving_jobs(Y, np, len(scenes), n_jobs, args.rate if serve else 0.0)
     lat, done_t = [], []
-    with Y.Context(mem_budget=64 << 30) as ctx:
+    with Y.Context(mem_budget=64 << 30, graph=args.graph, trace_overlap=args.trace_overlap) as ctx:
         t0 = time.perf_counter()
         sids = [ctx.scene(s) for s in scenes]
         print(f"# {len(scenes)} scenes, {sum(len(s.tris) for s in scenes)} triangles, uploaded in "
