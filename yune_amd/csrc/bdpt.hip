@@ -210,43 +210,47 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
     }
 
     bool ended = false;
+    // Subpath steps and the camera ray either make a ray or hand over to a
+    // later step; only the connection step goes back (to the next eye
+    // vertex's round), so only the round steps sit in a loop and the
+    // subpath vertex is not kept live through them.
+    if (step == T_LIGHT_NEXT || step == T_EYE_NEXT) do {   // :474-484, :532-544
+        const bool light = step == T_LIGHT_NEXT;
+        Mat m = load_mat(S.mats, S.tri_mat[cur.tri]);
+        float prob = 0.0f, pdf = 1.0f;
+        bool g = sample_glossy_pdf(m, seed, prob, true);
+        Step stop = light ? T_EYE_START : T_ROUND_START;
+        if (prob == 0.0f) { step = stop; if (!light) { k = 0; w_eye = 1.0f; color = mk(0, 0, 0, 1); } break; }
+        f4 d = g ? phong_sample(m, pdf, -cur.d, cur.p, cur.n, seed, true) : cosine_sample(pdf, cur.p, cur.n, seed);
+        if (pdf <= 0.0f) { step = stop; if (!light) { k = 0; w_eye = 1.0f; color = mk(0, 0, 0, 1); } break; }
+        f4 o = cur.p + d * kEps;
+        float tq, tmax = INFINITY;
+        st &= ~(BF_LCAND | BF_BGLOSSY);
+        if (quad_hit(L, o, d, INFINITY, tq)) { tmax = tq; st |= BF_LCAND; }
+        if (g) st |= BF_BGLOSSY;
+        P.nd[p] = st4(d);
+        P.bnc[p] = make_float4(pdf, prob, 0.0f, 0.0f);
+        st = (st & ~B_MASK) | (light ? B_LIGHT : B_EYE);
+        k = k + 1;
+        out.ext = true;
+        set_ray(out, o, d, tmax, p);
+        goto emitted;
+    } while (0);
+    if (step == T_EYE_START) do {          // the camera ray (createEyePath :517)
+        f4 o = ld(B.eye_o[p]), d = ld(B.eye_d[p]);
+        float tq, tmax = INFINITY;
+        st &= ~(BF_LCAND | BF_BGLOSSY);
+        if (quad_hit(L, o, d, INFINITY, tq)) { tmax = tq; st |= BF_LCAND; }
+        st = (st & ~B_MASK) | B_EYE;
+        k = 0;
+        out.ext = true;
+        set_ray(out, o, d, tmax, p);
+        goto emitted;
+    } while (0);
     // a no-ray round is 3 steps per eye vertex (<= 64): the bound is never reached
-    for (int guard = 0; guard < 1024 && !ended; ++guard) {
-        if (step == T_LIGHT_NEXT || step == T_EYE_NEXT) {   // :474-484, :532-544
-            const bool light = step == T_LIGHT_NEXT;
-            Mat m = load_mat(S.mats, S.tri_mat[cur.tri]);
-            float prob = 0.0f, pdf = 1.0f;
-            bool g = sample_glossy_pdf(m, seed, prob, true);
-            Step stop = light ? T_EYE_START : T_ROUND_START;
-            if (prob == 0.0f) { step = stop; if (!light) { k = 0; w_eye = 1.0f; color = mk(0, 0, 0, 1); } continue; }
-            f4 d = g ? phong_sample(m, pdf, -cur.d, cur.p, cur.n, seed, true) : cosine_sample(pdf, cur.p, cur.n, seed);
-            if (pdf <= 0.0f) { step = stop; if (!light) { k = 0; w_eye = 1.0f; color = mk(0, 0, 0, 1); } continue; }
-            f4 o = cur.p + d * kEps;
-            float tq, tmax = INFINITY;
-            st &= ~(BF_LCAND | BF_BGLOSSY);
-            if (quad_hit(L, o, d, INFINITY, tq)) { tmax = tq; st |= BF_LCAND; }
-            if (g) st |= BF_BGLOSSY;
-            P.nd[p] = st4(d);
-            P.bnc[p] = make_float4(pdf, prob, 0.0f, 0.0f);
-            st = (st & ~B_MASK) | (light ? B_LIGHT : B_EYE);
-            k = k + 1;
-            out.ext = true;
-            set_ray(out, o, d, tmax, p);
-            break;
-        }
-        if (step == T_EYE_START) {          // the camera ray (createEyePath :517)
-            f4 o = ld(B.eye_o[p]), d = ld(B.eye_d[p]);
-            float tq, tmax = INFINITY;
-            st &= ~(BF_LCAND | BF_BGLOSSY);
-            if (quad_hit(L, o, d, INFINITY, tq)) { tmax = tq; st |= BF_LCAND; }
-            st = (st & ~B_MASK) | B_EYE;
-            k = 0;
-            out.ext = true;
-            set_ray(out, o, d, tmax, p);
-            break;
-        }
-        if (step == T_ROUND_START) {        // eye vertex k: NEE start + connection rays
-            if (k >= ep_len || w_eye == 0) { step = T_FINISH; continue; }
+    for (int guard = 0; guard < 1024; ++guard) {
+        if (step == T_ROUND_START) do {        // eye vertex k: NEE start + connection rays
+            if (k >= ep_len || w_eye == 0) { step = T_FINISH; break; }
             Vtx e = load_v(B, 1, k, p);
             Mat m = load_mat(S.mats, S.tri_mat[e.tri]);
             conn = 0;
@@ -291,16 +295,16 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
                 float tq;
                 if (conn_ray(e, l, o, d, cl) && !quad_hit(L, o, d, cl, tq)) conn |= 1u << j;
             }
-            if (conn == 0) { step = T_NEE_SHADOW; continue; }
+            if (conn == 0) { step = T_NEE_SHADOW; break; }
             st = (st & ~B_MASK) | B_ROUND;
             out.emit = conn;
             out.eye_i = k;
-            break;
-        }
-        if (step == T_NEE_SHADOW) {         // udpt.cl:557-597 (bdpt variant of the BRDF pieces)
+            goto emitted;
+        } while (0);
+        if (step == T_NEE_SHADOW) do {         // udpt.cl:557-597 (bdpt variant of the BRDF pieces)
             Vtx e = load_v(B, 1, k, p);
             Mat m = load_mat(S.mats, S.tri_mat[e.tri]);
-            if (st & BF_NEE_DONE) { nee_res = ld(P.ls[p]); step = T_CONNECT; continue; }
+            if (st & BF_NEE_DONE) { nee_res = ld(P.ls[p]); step = T_CONNECT; break; }
             int occ = (st & BF_NEE_Q) ? 1 : B.occ[((size_t)p << B.shift)];
             if (cap_on && k == 0 && !(st & BF_CAPSH)) { cap.shadow_occ[pix] = occ; st |= BF_CAPSH; }
             float4 w4 = P.wi[p];
@@ -314,20 +318,20 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
             if (!occ) {
                 if (!fix) {
                     glossy = sample_glossy_pdf(m, seed, nprob, true);
-                    if (nprob == 0.0f) { nee_res = em; step = T_CONNECT; continue; }
+                    if (nprob == 0.0f) { nee_res = em; step = T_CONNECT; break; }
                 }
                 ls = (eval_brdf(m, wi, w_o, e.p, e.n, glossy, nprob, false, true) * ld(L.ke)) * fmaxf(dot(wi, e.n), 0.0f);
                 ls = ls * (1 / lpdf);
             }
-            if (!RP.mis) { nee_res = ls + em; step = T_CONNECT; continue; }
+            if (!RP.mis) { nee_res = ls + em; step = T_CONNECT; break; }
             float bpdf = glossy ? phong_pdf(m, wi, w_o, e.n) : cos_pdf(wi, e.n);
             float mw = power_heuristic(lpdf, lpdf, bpdf);
             ls = ls * mw;
             f4 bd = glossy ? phong_sample(m, bpdf, w_o, e.p, e.n, seed, true) : cosine_sample(bpdf, e.p, e.n, seed);
-            if (bpdf <= 0.0f) { nee_res = ls + em; step = T_CONNECT; continue; }
+            if (bpdf <= 0.0f) { nee_res = ls + em; step = T_CONNECT; break; }
             f4 bo = e.p + bd * kEps;
             float tq;
-            if (!quad_hit(L, bo, bd, INFINITY, tq)) { nee_res = ls + em; step = T_CONNECT; continue; }
+            if (!quad_hit(L, bo, bd, INFINITY, tq)) { nee_res = ls + em; step = T_CONNECT; break; }
             st = glossy ? (st | BF_NGLOSSY) : (st & ~BF_NGLOSSY);
             P.ls[p] = st4(ls);
             P.mis[p] = make_float4(bd.x, bd.y, bd.z, bpdf);
@@ -337,15 +341,15 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
             out.emit = 1u;
             out.eye_i = k;
             set_ray(out, bo, bd, tq, p << B.shift);
-            break;
-        }
-        if (step == T_NEE_MIS) {            // udpt.cl:599-607
+            goto emitted;
+        } while (0);
+        if (step == T_NEE_MIS) do {            // udpt.cl:599-607
             Vtx e = load_v(B, 1, k, p);
             Mat m = load_mat(S.mats, S.tri_mat[e.tri]);
             f4 em = m.ke;
             f4 ls = ld(P.ls[p]);
             int occ = B.occ[((size_t)p << B.shift)];
-            if (occ) { nee_res = ls + em; step = T_CONNECT; continue; }
+            if (occ) { nee_res = ls + em; step = T_CONNECT; break; }
             float4 mb = P.mis[p];
             f4 bd = mk(mb.x, mb.y, mb.z, 0.0f);
             float bpdf = mb.w;
@@ -357,9 +361,10 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
             bs = bs * (mw / bpdf);
             nee_res = (ls + bs) + em;
             step = T_CONNECT;
-            continue;
-        }
-        if (step == T_CONNECT) {            // shading :578-637 for eye vertex k
+            break;
+        } while (0);
+        if (step != T_CONNECT) break;
+        {            // shading :578-637 for eye vertex k
             Vtx e = load_v(B, 1, k, p);
             Mat m = load_mat(S.mats, S.tri_mat[e.tri]);
             f4 spec = m.ks, em = m.ke;
@@ -393,9 +398,10 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
             step = T_ROUND_START;
             continue;
         }
-        if (step == T_FINISH) { write_sample(J, slot, pix, color); ended = true; break; }
-        if (step == T_FINISH_CONST) { write_sample(J, slot, pix, const_color); ended = true; break; }
     }
+    if (step == T_FINISH) { write_sample(J, slot, pix, color); ended = true; }
+    else if (step == T_FINISH_CONST) { write_sample(J, slot, pix, const_color); ended = true; }
+emitted:
     if (!out.ext && out.emit == 0 && !ended) {   // unreachable by construction; never leave a path hanging
         write_sample(J, slot, pix, mk(NAN, NAN, NAN, NAN));
         ended = true;
