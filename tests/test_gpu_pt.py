@@ -19,6 +19,8 @@ Tolerances:
     every other float operation rounds once in the reference's order, so
     only closest-hit ties (equal t on two triangles) can make paths differ.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -328,3 +330,25 @@ def test_graph_replay_keeps_images(mode):
             imgs.append([j.read() for j in jobs])
     for a, b in zip(*imgs):
         assert np.array_equal(a.view(np.int32), b.view(np.int32))
+
+
+def test_nan_rays_hit_nothing_like_the_oracle():
+    """Rays with a NaN in origin, direction or length: every triangle test
+    rejects them (NaN u, v or t), so both kinds of query answer "no hit",
+    as the oracle's traversal does; the trace answers them without walking
+    the tree (the reference's NaN guard would keep every box)."""
+    s = Y.gen_icosphere_room(11, 6, 3)
+    rays = random_rays(512, 3, (-4.9, -4.9, -4.9), (4.9, 4.9, 4.9))
+    for j, i in enumerate(range(0, 512, 8)):
+        rays[i, j % 7] = np.nan          # o.x, o.y, o.z, length, d.x, d.y, d.z in turn
+    nan_rows = np.isnan(rays[:, :7]).any(1)
+    assert nan_rows.sum() >= 60
+    with Y.Context(max_paths=1 << 14) as c:
+        t0 = time.perf_counter()
+        gtri, glight, gt = c.trace(c.scene(s), rays)
+        assert time.perf_counter() - t0 < 10
+    otri, olight, ot, _ = oscene(s, s.bvh()).trace(rays)
+    assert (gtri[nan_rows] == -1).all() and (otri[nan_rows] == -1).all()
+    assert np.array_equal(glight[nan_rows], olight[nan_rows])
+    ok = ~nan_rows
+    assert (gtri[ok] == otri[ok]).mean() > 0.99

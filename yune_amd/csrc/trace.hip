@@ -335,8 +335,17 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
             // only an infinite dir_inv (a zero or denormal direction
             // component) can make the reference's NaN guard matter
             bool inf_inv = fabsf(r.ix) == INFINITY || fabsf(r.iy) == INFINITY || fabsf(r.iz) == INFINITY;
-            bool got = __ballot(inf_inv) ? traverse<W, ANY, LST, BLOCK, COUNT, true>(S, r, st, c)
-                                          : traverse<W, ANY, LST, BLOCK, COUNT, false>(S, r, st, c);
+            // A ray with a NaN in its origin, direction or length hits nothing:
+            // u, v or t of every triangle test is NaN, and each acceptance
+            // comparison is false.  Its traversal would still visit every
+            // node (the NaN guard keeps each slab), one lane for the whole
+            // tree, so it is answered without one.
+            bool nan_ray = !(ot.x == ot.x && ot.y == ot.y && ot.z == ot.z && ot.w == ot.w && dp.x == dp.x &&
+                             dp.y == dp.y && dp.z == dp.z);
+            bool got = false;
+            if (!nan_ray)
+                got = __ballot(inf_inv) ? traverse<W, ANY, LST, BLOCK, COUNT, true>(S, r, st, c)
+                                        : traverse<W, ANY, LST, BLOCK, COUNT, false>(S, r, st, c);
             ++nrays;
             if (r.overflow) atomicOr(&a.ctr->q[7], 1u);
             if (ANY) {
