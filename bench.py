@@ -3,16 +3,16 @@
 Default workload (the headline): the ~1M-triangle 1920x1080 scene (48
 icospheres of 20,480 triangles in a 10x10x10 room, 2-triangle quad light),
 path tracing with NEE+MIS, max depth 8, Russian roulette from bounce 3.
-A step is one batch of the wavefront path tracer: 8 frames of 1920x1080 at
-1 sample per pixel each (16.6M paths) run from primary rays until every path
-has ended, then folded into the progressive image; 8 steps are the 64 spp
-of the workload.
+A step is one batch of the wavefront path tracer: 16 frames of 1920x1080
+at 1 sample per pixel each (33.2M paths) run from primary rays until every
+path has ended, then folded into the progressive image; 4 steps are the 64
+spp of the workload.
 
 Other single-GPU workloads (--workload):
   cornell  the 36-triangle Cornell box, 512x512, NEE+MIS, depth 8, RR from
-           bounce 3; 32 frames per step, 32 steps = 1024 spp
+           bounce 3; 128 frames per step, 8 steps = 1024 spp
   bdpt     bidirectional (bdpt.cl semantics) on the Cornell box, 1280x720,
-           5-vertex subpaths, MIS; 4 frames per step, 8 steps = 32 spp
+           5-vertex subpaths, MIS; 8 frames per step, 4 steps = 32 spp
 
   batch    64 jobs on 8 icosphere scenes (20k-2M triangles), random cameras,
            128^2-1024^2, 16-256 spp, all submitted at once to the scheduler
@@ -41,17 +41,17 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_ext_1m_1080p_traffic.txt"
 
 WORKLOADS = {
     "icosphere": dict(
-        metric="Mrays/s at 1M tris 1080p", W=1920, H=1080, frames=8, steps=8, mode=0,
+        metric="Mrays/s at 1M tris 1080p", W=1920, H=1080, frames=16, steps=4, mode=0,
         params=dict(mis=1, max_bounces=8, rr_threshold=2),
         desc="path tracing, 48 icospheres x 20480 tris + 12-tri room + 2-tri quad light ({n} tris), 1920x1080, "
              "{f} spp per step, max depth 8, RR from bounce 3, NEE+MIS"),
     "cornell": dict(
-        metric="Mrays/s Cornell box 512x512", W=512, H=512, frames=32, steps=32, mode=0,
+        metric="Mrays/s Cornell box 512x512", W=512, H=512, frames=128, steps=8, mode=0,
         params=dict(mis=1, max_bounces=8, rr_threshold=2),
         desc="path tracing, Cornell box ({n} tris), 512x512, {f} spp per step, max depth 8, RR from bounce 3, "
              "NEE+MIS"),
     "bdpt": dict(
-        metric="Mrays/s BDPT Cornell box 720p", W=1280, H=720, frames=4, steps=8, mode=1,
+        metric="Mrays/s BDPT Cornell box 720p", W=1280, H=720, frames=8, steps=4, mode=1,
         params=dict(mis=1, bdpt_bounces=5),
         desc="bidirectional path tracing (bdpt.cl), Cornell box ({n} tris), 1280x720, {f} spp per step, "
              "5-vertex eye and light subpaths, all connections, NEE+MIS"),

@@ -236,7 +236,7 @@ int alloc_wavefront(yn_ctx* c, int64_t n)
     return 0;
 }
 
-constexpr size_t kBdptBudget = (size_t)4 << 30;   // bytes for bidirectional path state
+constexpr size_t kBdptBudget = (size_t)16 << 30;  // bytes for bidirectional path state
 constexpr int kMaxBdptBounces = 64;
 
 void free_bdpt(yn_ctx* c)
@@ -923,7 +923,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     *out = nullptr;
     yn_config k{};
     if (cfg) k = *cfg;
-    if (k.max_paths <= 0) k.max_paths = 1 << 24;
+    if (k.max_paths <= 0) k.max_paths = 1 << 25;
     if (k.mem_budget == 0) k.mem_budget = (size_t)64 << 30;
     if (k.bvh_width == 0) k.bvh_width = 4;
     if (k.trace_block == 0) k.trace_block = 256;
