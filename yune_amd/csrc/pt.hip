@@ -74,8 +74,9 @@ __device__ __forceinline__ Out make_ray(int kind, f4 o, f4 d, float tmax, int p)
 
 // One shade step of path p.  The ray whose result arrived is rebuilt from
 // the path state, bit for bit as it was made: a camera ray starts at the
-// camera's eye (createRay, udpt.cl:236) along P.rd; a bounce ray at
-// hp + nd * EPSILON along P.nd (udpt.cl:766, 906, 945).
+// camera's eye (createRay, udpt.cl:236) along the stored direction rd; a
+// bounce ray at hp + nd * EPSILON along the pending bounce direction nd
+// (udpt.cl:766, 906, 945).
 __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
 {
     const PathSoA& P = a.P;

@@ -196,11 +196,7 @@ int alloc_wavefront(yn_ctx* c, int64_t n)
     };
     PathSoA& P = c->P;
     int r = 0;
-    r |= A(&P.seed); r |= A(&P.pix); r |= A(&P.job); r |= A(&P.slot); r |= A(&P.state);
-    r |= A(&P.bounce); r |= A(&P.tri);
-    r |= A(&P.hp); r |= A(&P.hn); r |= A(&P.rd); r |= A(&P.tp); r |= A(&P.direct); r |= A(&P.indirect);
-    r |= A(&P.nd); r |= A(&P.bnc); r |= A(&P.wi); r |= A(&P.ls); r |= A(&P.mis); r |= A(&P.nee_prob);
-    r |= A(&P.hit); r |= A(&P.occ);
+    r |= A(&P.job); r |= A(&P.hit); r |= A(&P.occ);
     {
         auto B = [&](auto** q, size_t k) -> int {
             int rr = dalloc(c, q, (size_t)n * k);
@@ -246,6 +242,10 @@ void free_bdpt(yn_ctx* c)
     c->B = BdptSoA{};
     c->ShB[0] = c->ShB[1] = RayQueue{};
     c->lst[0] = c->lst[1] = nullptr;
+    PathSoA& P = c->P;
+    P.seed = nullptr; P.pix = nullptr; P.slot = nullptr; P.state = nullptr; P.bounce = nullptr;
+    P.direct = nullptr; P.nd = nullptr; P.bnc = nullptr; P.wi = nullptr; P.ls = nullptr; P.mis = nullptr;
+    P.nee_prob = nullptr;
     c->cap_bdpt = 0;
 }
 
@@ -261,7 +261,9 @@ int ensure_bdpt(yn_ctx* c, int bounces)
     int shift = 0;
     while ((1 << shift) < K) ++shift;
     const int64_t R = (int64_t)1 << shift;
-    const int64_t per_path = 2 * K * 64 + R * 4 + 2 * R * 32 + 2 * 4 + 3 * 4 + 2 * 16;
+    // vertices, shadow results and queues, path lists, subpath bookkeeping,
+    // and the per-path shading fields of PathSoA used by this mode
+    const int64_t per_path = 2 * K * 64 + R * 4 + 2 * R * 32 + 2 * 4 + 3 * 4 + 2 * 16 + 6 * 4 + 6 * 16;
     int64_t cap = std::min<int64_t>(c->cap_paths, (int64_t)(kBdptBudget / (size_t)per_path));
     cap = std::min<int64_t>(cap, ((int64_t)1 << (31 - shift)) - 1);
     if (cap < 1) return fail(c, YN_ERR_NO_MEMORY, "no room for bidirectional paths");
@@ -281,6 +283,10 @@ int ensure_bdpt(yn_ctx* c, int bounces)
         r |= A(&c->ShB[k].o_t, ns); r |= A(&c->ShB[k].d_p, ns);
         r |= A(&c->lst[k], n);
     }
+    PathSoA& P = c->P;
+    r |= A(&P.seed, n); r |= A(&P.pix, n); r |= A(&P.slot, n); r |= A(&P.state, n); r |= A(&P.bounce, n);
+    r |= A(&P.direct, n); r |= A(&P.nd, n); r |= A(&P.bnc, n); r |= A(&P.wi, n); r |= A(&P.ls, n);
+    r |= A(&P.mis, n); r |= A(&P.nee_prob, n);
     if (r) { free_bdpt(c); return r < 0 ? r : YN_ERR_HIP; }
     B.K = K;
     B.shift = shift;

@@ -49,27 +49,22 @@ struct RayQueue {
 
 // Path states, structure of arrays, one entry per path in flight.
 struct PathSoA {
+    int32_t* job;               // job slot (read by the trace kernels' scene lookup)
+    float4* hit;                // closest-hit result: t, tri bits, u, v
+    int32_t* occ;               // any-hit result
+    // bidirectional mode only (allocated with its state, one entry per path):
     uint32_t* seed;
     int32_t* pix;               // pixel index y*W + x
-    int32_t* job;               // job slot
     int32_t* slot;              // frame slot within the job's sample buffer
     int32_t* state;             // state | flags
-    int32_t* bounce;            // bounce index i (udpt.cl:463)
-    int32_t* tri;               // triangle of the current vertex
-    float4* hp;                 // current vertex position
-    float4* hn;                 // current vertex shading normal
-    float4* rd;                 // direction of the ray that found the current vertex
-    float4* tp;                 // throughput
-    float4* direct;
-    float4* indirect;
+    int32_t* bounce;            // vertex index of the subpath being built
+    float4* direct;             // colour accumulated so far
     float4* nd;                 // pending bounce direction
-    float4* bnc;                // pending bounce: pdf, brdf_prob, ior_factor, -
+    float4* bnc;                // pending bounce: pdf, brdf_prob, -, -
     float4* wi;                 // NEE: normalised w_i, w = light pdf
     float4* ls;                 // NEE: light sample (MIS-weighted)
     float4* mis;                // NEE: MIS brdf dir xyz, w = brdf pdf
     float* nee_prob;            // NEE: lobe probability of the light sample
-    float4* hit;                // closest-hit result: t, tri bits, u, v
-    int32_t* occ;               // any-hit result
     // Unidirectional shading state, grouped by use: a shade step reads and
     // writes whole 32/64-byte records instead of one cache line per field,
     // which matters because a queue's paths are in no particular slot order.
