@@ -65,7 +65,8 @@ typedef struct yn_config {
                                  run on two streams and overlap; 0: one stream     */
     int32_t graph;            /* 1: groups of bounce iterations are captured once in
                                  a HIP graph and replayed; 0: launched one by one  */
-    int32_t pad;
+    int32_t trace_xcd;        /* ray-queue slices of a trace launch, one per XCD with
+                                 its own work counter (1..16; 0: 1, one queue)     */
 } yn_config;
 
 /* Parameters of udpt.cl / bdpt.cl as shipped: GI on, MIS off, no bounce
@@ -221,7 +222,8 @@ double yn_clock_ms(yn_ctx* ctx);                     /* ms since yn_create */
  * timed runs of the given view (frames per run) and keeps, knob by knob,
  * the fastest BVH width (the scenes' GPU layout is rebuilt), trace block
  * size, rays per queue fetch, LDS stack depth, shade occupancy, paths
- * per batch, batch_factor, trace_overlap and graph in the context's configuration.  Not while
+ * per batch, batch_factor, trace_overlap, graph and trace_xcd in the context's
+ * configuration.  Not while
  * served jobs are pending.  yn_config_get reads the configuration in effect. ---- */
 typedef struct yn_tune_result {
     int32_t bvh_width, trace_block, trace_chunk, lds_stack, shade_waves, n_tried;
@@ -230,7 +232,7 @@ typedef struct yn_tune_result {
                                  paths' slots are regenerated from the remaining work */
     int32_t trace_overlap;    /* closest-hit and any-hit traces on two streams        */
     int32_t graph;            /* bounce loop replayed from a HIP graph                */
-    int32_t pad;
+    int32_t trace_xcd;        /* ray-queue slices per trace launch                    */
     double  mrays_per_s;      /* the chosen configuration on the tuning view */
 } yn_tune_result;
 int  yn_autotune(yn_ctx* ctx, int32_t scene_id, const yn_cam* cam, int32_t width, int32_t height,

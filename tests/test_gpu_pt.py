@@ -352,3 +352,29 @@ def test_nan_rays_hit_nothing_like_the_oracle():
     assert np.array_equal(glight[nan_rows], olight[nan_rows])
     ok = ~nan_rows
     assert (gtri[ok] == otri[ok]).mean() > 0.99
+
+
+@pytest.mark.parametrize("mode", [Y.YN_MODE_PT, Y.YN_MODE_BDPT])
+def test_trace_queue_slices_keep_images(mode):
+    """Ray queues cut into slices (one work counter per XCD, waves moving on
+    to the next slices once theirs is used up) trace the same rays: images
+    are bit-identical for 1, 3 and 8 slices, and so are the trace results of
+    caller rays."""
+    s = Y.gen_icosphere_room(3, 9, 2)
+    W, H, F = 64, 48, 3
+    kw = dict(bdpt_bounces=4) if mode else dict(mis=1, max_bounces=8, rr_threshold=2)
+    p = Y.params_reference(mode, **kw)
+    rays = random_rays(12000, 8, (-4.9, -4.9, -4.9), (4.9, 4.9, 4.9))     # within the 16k paths
+    imgs, traced = [], []
+    for ns in (1, 3, 8):
+        with Y.Context(max_paths=1 << 14, trace_xcd=ns) as c:
+            assert c.config()["trace_xcd"] == ns
+            sid = c.scene(s)
+            job = c.job(sid, s.cam, W, H, p, base_seed=29)
+            job.render(F)
+            imgs.append(job.read())
+            traced.append(c.trace(sid, rays))
+    for k in (1, 2):
+        assert np.array_equal(imgs[0].view(np.int32), imgs[k].view(np.int32))
+        for a, b in zip(traced[0], traced[k]):
+            assert np.array_equal(np.asarray(a).view(np.int32), np.asarray(b).view(np.int32))

@@ -422,7 +422,7 @@ emitted:
 __global__ __launch_bounds__(256) void k_shade_bdpt(BdptShadeArgs a, DevCapture cap)
 {
     const uint32_t n = *a.list_in_count;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { *a.zero0 = 0; *a.zero1 = 0; }
+    if (blockIdx.x == 0 && (int)threadIdx.x < a.n_zero) a.zero[threadIdx.x] = 0;
     const int lane = (int)__lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;

@@ -375,7 +375,7 @@ __global__ __launch_bounds__(256, WAVES) void k_shade_pt(ShadeArgs a, DevCapture
 {
     const uint32_t n_ext = *a.ext_in.count, n_sh = *a.sh_in.count;
     const uint32_t total = n_ext + n_sh;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { *a.zero0 = 0; *a.zero1 = 0; }
+    if (blockIdx.x == 0 && (int)threadIdx.x < a.n_zero) a.zero[threadIdx.x] = 0;
     const int lane = (int)__lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;

@@ -368,7 +368,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     const DevScene* scenes = c->d_scenes;
     uint32_t* q = c->ctr->q;
     // counters: q[0],q[1] extension queue counts, q[2],q[3] shadow queue counts,
-    // q[4],q[5] work counters of the extend / shadow launches, q[7] stack
+    // q[4],q[5] unused, q[7] stack
     // overflow flag, q[8],q[9] path-list counts (bidirectional mode), q[10]
     // free slots (regeneration).
     for (int k = 0; k < 2; ++k) {
@@ -447,7 +447,8 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         // extension rays
         ta.q = c->E[cur];
         ta.hit_out = c->P.hit;
-        ta.work = q + 4;
+        ta.work = c->ctr->work[0];
+        ta.n_slices = c->cfg.trace_xcd;
         ta.zero0 = c->E[nxt].count;
         ta.zero1 = SQ[nxt].count;
         // shadow rays
@@ -457,7 +458,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         ts.hit_out = nullptr;
         ts.occ_out = bdpt ? c->B.occ : c->P.occ;
         ts.id_shift = bdpt ? c->B.shift : 0;
-        ts.work = q + 5;
+        ts.work = c->ctr->work[1];
         ts.zero0 = bdpt ? q + 8 + nxt : nullptr;    // the next path list
         ts.zero1 = nullptr;
         if (overlap) {          // the second half of the overflow area
@@ -476,8 +477,8 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             ba.list_out_count = q + 8 + nxt;
             ba.ext_out = c->E[nxt];
             ba.sh_out = c->ShB[nxt];
-            ba.zero0 = q + 4;
-            ba.zero1 = q + 5;
+            ba.zero = &c->ctr->work[0][0];
+            ba.n_zero = 2 * kMaxXcd;
         } else {
             ShadeArgs& sa = A.sa;
             sa.scenes = scenes;
@@ -487,8 +488,8 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             sa.sh_in = c->Sh[cur];
             sa.ext_out = c->E[nxt];
             sa.sh_out = c->Sh[nxt];
-            sa.zero0 = q + 4;
-            sa.zero1 = q + 5;
+            sa.zero = &c->ctr->work[0][0];
+            sa.n_zero = 2 * kMaxXcd;
             sa.ctr = c->ctr;
             sa.free_list = regen ? c->free_list : nullptr;
             sa.free_count = q + 10;
@@ -946,6 +947,8 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.batch_factor != 1 && k.batch_factor != 2) return fail(nullptr, YN_ERR_INVALID, "batch_factor must be 1 or 2");
     if (k.trace_overlap != 0 && k.trace_overlap != 1) return fail(nullptr, YN_ERR_INVALID, "trace_overlap must be 0 or 1");
     if (k.graph != 0 && k.graph != 1) return fail(nullptr, YN_ERR_INVALID, "graph must be 0 or 1");
+    if (k.trace_xcd == 0) k.trace_xcd = 1;
+    if (k.trace_xcd < 1 || k.trace_xcd > kMaxXcd) return fail(nullptr, YN_ERR_INVALID, "trace_xcd must be 1..16");
     if (k.bvh_width != 2 && k.bvh_width != 4) return fail(nullptr, YN_ERR_INVALID, "bvh_width must be 2 or 4");
     if (k.trace_block != 64 && k.trace_block != 128 && k.trace_block != 256)
         return fail(nullptr, YN_ERR_INVALID, "trace_block must be 64, 128 or 256");
@@ -1375,6 +1378,7 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         {6, {1, 2}},                            // work per batch over paths in flight (regeneration)
         {7, {0, 1}},                            // overlapping closest-hit and any-hit traces
         {8, {0, 1}},                            // bounce loop replayed from a graph
+        {9, {1, 8}},                            // queue slices of the traces (one per XCD)
     };
     auto get = [&](int kind) -> int64_t {
         switch (kind) {
@@ -1386,7 +1390,8 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         case 5: return c->batch_paths > 0 ? c->batch_paths : cap;
         case 6: return c->cfg.batch_factor;
         case 7: return c->cfg.trace_overlap;
-        default: return c->cfg.graph;
+        case 8: return c->cfg.graph;
+        default: return c->cfg.trace_xcd;
         }
     };
     auto set = [&](int kind, int64_t v) -> int {
@@ -1399,7 +1404,8 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         case 5: c->batch_paths = v; return 0;
         case 6: c->cfg.batch_factor = (int)v; return 0;
         case 7: c->cfg.trace_overlap = (int)v; return 0;
-        default: c->cfg.graph = (int)v; return 0;
+        case 8: c->cfg.graph = (int)v; return 0;
+        default: c->cfg.trace_xcd = (int)v; return 0;
         }
     };
     double best = measure();
@@ -1429,6 +1435,7 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
     out->batch_factor = c->cfg.batch_factor;
     out->trace_overlap = c->cfg.trace_overlap;
     out->graph = c->cfg.graph;
+    out->trace_xcd = c->cfg.trace_xcd;
     out->mrays_per_s = best / 1e6;
     out->n_tried = tried;
     return 0;
@@ -1550,7 +1557,8 @@ int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri,
         ta.q = q;
         ta.hit_out = c->P.hit;
         ta.occ_out = any ? c->P.occ : nullptr;
-        ta.work = c->ctr->q + 4;
+        ta.work = c->ctr->work[0];
+        ta.n_slices = c->cfg.trace_xcd;
         int grid = c->n_cu * (2048 / c->cfg.trace_block);
         if (int rr = ensure_overflow(c, S.info.max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * c->cfg.trace_block))
             return rr;

@@ -301,21 +301,49 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
     const uint32_t n = *a.q.count;
     Counts c{0, 0};
     unsigned nrays = 0;
-    // chunk `wave` first without an atomic, then chunks past the static ones
-    // from the shared counter: a wave with nothing to do leaves without
-    // touching it (same-address atomics from every wave of the persistent
-    // grid would otherwise set a floor on the launch time of short queues).
-    const uint32_t wave = gtid >> 6;
-    const uint32_t n_static = (gridDim.x * BLOCK) >> 6;
+    // The queue is cut into one slice of chunks per XCD: workgroups go to
+    // the XCDs round-robin, so block b runs on XCD b % n_slices and takes
+    // its chunks from slice b % n_slices, where neighbouring rays (neighbouring
+    // pixels) share that XCD's L2.  Each wave first takes its static chunk of
+    // its slice without an atomic, then chunks past the slice's static ones
+    // from the slice's counter; when the slice is used up it moves on to the
+    // next slices in turn (never back), so every chunk is taken exactly once
+    // and every wave ends after at most n_slices empty requests.  When the
+    // static chunks cover the whole queue, a wave leaves without touching a
+    // counter (same-address atomics from every wave would otherwise set a
+    // floor on the launch time of short queues).
+    const uint32_t ns = (uint32_t)a.n_slices;
+    const uint32_t G = gridDim.x, wpb = BLOCK >> 6;
+    const uint32_t xs = blockIdx.x % ns;
+    const uint32_t wx = (blockIdx.x / ns) * wpb + (threadIdx.x >> 6);     // wave within its slice's XCD
+    const uint32_t chunk = (uint32_t)a.chunk;
+    const uint32_t n_chunks = (n + chunk - 1) / chunk;
+    auto waves_of = [&](uint32_t y) { return ((G - y + ns - 1) / ns) * wpb; };
+    auto slice_lo = [&](uint32_t y) { return (uint32_t)(((uint64_t)y * n_chunks) / ns); };
+    const bool all_static = n_chunks <= ns * ((G / ns) * wpb);
+    uint32_t y = xs, moved = 0;
     for (bool first = true;; first = false) {
-        uint32_t base = 0;
-        if (first) {
-            base = wave * (uint32_t)a.chunk;
-        } else {
-            if (lane == 0) base = atomicAdd(a.work, (uint32_t)a.chunk);
-            base = __shfl(base, 0) + n_static * (uint32_t)a.chunk;
+        uint32_t ci;
+        for (;;) {
+            const uint32_t c0 = slice_lo(y), c1 = slice_lo(y + 1);
+            if (first) {
+                ci = c0 + wx;
+            } else if (all_static) {
+                ci = n_chunks;          // nothing past the static chunks
+                break;
+            } else {
+                uint32_t k = 0;
+                if (lane == 0) k = atomicAdd(a.work + y, 1u);
+                ci = c0 + waves_of(y) + __shfl(k, 0);
+            }
+            if (ci < c1) break;
+            if (all_static) { ci = n_chunks; break; }
+            first = false;
+            if (++moved >= ns) { ci = n_chunks; break; }
+            y = (y + 1) % ns;
         }
-        if (base >= n) break;
+        if (ci >= n_chunks) break;
+        const uint32_t base = ci * chunk;
         for (uint32_t off = (uint32_t)lane; off < (uint32_t)a.chunk; off += 64) {
             uint32_t i = base + off;
             if (i >= n) break;

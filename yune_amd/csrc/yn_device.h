@@ -95,8 +95,11 @@ struct BdptSoA {
 };
 
 // Counters and stats in device memory.
+constexpr int kMaxXcd = 16;     // work counters per trace launch (one per queue slice)
+
 struct DevCounters {
     uint32_t q[12];             // queue counts, work counters (see runtime)
+    uint32_t work[2][kMaxXcd];  // chunk counters of the extend / shadow traces, per queue slice
     unsigned long long rays_ext;
     unsigned long long rays_shadow;
     unsigned long long nodes_ext;     // counting mode only: interior nodes visited

@@ -17,7 +17,8 @@ struct TraceArgs {
     RayQueue q;
     float4* hit_out;            // per path: t, tri bits, u, v
     int32_t* occ_out;           // per path (any-hit)
-    uint32_t* work;             // work counter, zero at launch
+    uint32_t* work;             // chunk counters, one per queue slice, zero at launch
+    int32_t n_slices;           // queue slices (one per XCD, 1..kMaxXcd)
     uint32_t* zero0;            // counters this launch clears (next queues)
     uint32_t* zero1;
     DevCounters* ctr;
@@ -41,8 +42,8 @@ struct ShadeArgs {
     PathSoA P;
     RayQueue ext_in, sh_in;     // results to consume
     RayQueue ext_out, sh_out;   // rays to produce
-    uint32_t* zero0;            // work counters to clear for the next traces
-    uint32_t* zero1;
+    uint32_t* zero;             // work counters to clear for the next traces
+    int32_t n_zero;
     DevCounters* ctr;
     uint32_t* free_list;        // slots of ended paths (regeneration), or null
     uint32_t* free_count;
@@ -61,8 +62,8 @@ struct BdptShadeArgs {
     uint32_t* list_out;
     uint32_t* list_out_count;
     RayQueue ext_out, sh_out;
-    uint32_t* zero0;            // work counters to clear for the next traces
-    uint32_t* zero1;
+    uint32_t* zero;             // work counters to clear for the next traces
+    int32_t n_zero;
 };
 
 struct GenArgs {
