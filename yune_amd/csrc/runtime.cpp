@@ -367,10 +367,10 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
 
     const DevScene* scenes = c->d_scenes;
     uint32_t* q = c->ctr->q;
-    // counters: q[0],q[1] extension queue counts, q[2],q[3] shadow queue counts,
-    // q[4],q[5] unused, q[7] stack
-    // overflow flag, q[8],q[9] path-list counts (bidirectional mode), q[10]
-    // free slots (regeneration).
+    // counters: q[0],q[1] extension queue counts, q[2],q[3] shadow queue
+    // counts, q[7] stack overflow flag, q[8],q[9] path-list counts
+    // (bidirectional mode), q[10] free slots (regeneration); work[0], work[1]
+    // the chunk counters of the extend / shadow traces, per queue slice.
     for (int k = 0; k < 2; ++k) {
         c->E[k].count = q + k;
         c->Sh[k].count = q + 2 + k;
@@ -556,7 +556,6 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         }
     }
 
-    auto t0 = std::chrono::steady_clock::now();
     int it = 0;
     for (;;) {
         if (use_graph) {
@@ -604,7 +603,6 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         }
         if (empty) break;
     }
-    auto t1 = std::chrono::steady_clock::now();
     for (const Span& sp : spans) {
         float ms = 0.0f;
         hipEventElapsedTime(&ms, sp.a, sp.b);
@@ -622,7 +620,6 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     stat_job.st.nodes_shadow += c->h_ctr->nodes_shadow;
     stat_job.st.tris_shadow += c->h_ctr->tris_shadow;
     stat_job.st.samples += (uint64_t)total;
-    (void)t0; (void)t1;
     return 0;
 }
 
