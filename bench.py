@@ -46,8 +46,10 @@ WORKLOADS = {
         desc="path tracing, 48 icospheres x 20480 tris + 12-tri room + 2-tri quad light ({n} tris), 1920x1080, "
              "{f} spp per step, max depth 8, RR from bounce 3, NEE+MIS"),
     "cornell": dict(
-        # short rays: eight queue slices (one work counter per XCD), as yn_autotune picks for this scene
+        # short rays: eight queue slices (one work counter per XCD), waves run to their last ray, as
+        # yn_autotune picks for this scene
         metric="Mrays/s Cornell box 512x512", W=512, H=512, frames=128, steps=8, mode=0, trace_xcd=8,
+        trace_refill=-1,
         params=dict(mis=1, max_bounces=8, rr_threshold=2),
         desc="path tracing, Cornell box ({n} tris), 512x512, {f} spp per step, max depth 8, RR from bounce 3, "
              "NEE+MIS"),
@@ -265,6 +267,7 @@ def main():
     ap.add_argument("--trace-overlap", type=int, default=0, help="1: closest-hit and any-hit traces on two streams")
     ap.add_argument("--graph", type=int, default=0, help="1: bounce loop replayed from a HIP graph")
     ap.add_argument("--trace-xcd", type=int, default=0, help="ray-queue slices per trace launch (1: no slicing)")
+    ap.add_argument("--trace-refill", type=int, default=0, help="idle lanes before a trace wave takes new rays (-1: off)")
     ap.add_argument("--leaf-max", type=int, default=0, help="triangles per BVH leaf (0: library default)")
     ap.add_argument("--hoist-frac", type=float, default=0.0, help="box-area fraction above which triangles are hoisted")
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC counters")
@@ -300,12 +303,13 @@ def main():
                    trace_grid=args.trace_grid, lds_stack=args.lds_stack, shade_waves=args.shade_waves,
                    max_paths=args.max_paths, batch_factor=args.batch_factor,
                    trace_overlap=args.trace_overlap, graph=args.graph,
-                   trace_xcd=args.trace_xcd or wl.get("trace_xcd", 0)) as ctx:
+                   trace_xcd=args.trace_xcd or wl.get("trace_xcd", 0),
+                   trace_refill=args.trace_refill or wl.get("trace_refill", 0)) as ctx:
         sid = ctx.scene(scene, nodes)
         info = ctx.scene_info(sid)
         launch = {k: v for k, v in ctx.config().items()
                   if k in ("max_paths", "lds_stack", "shade_waves", "trace_block", "trace_chunk", "leaf_max",
-                           "trace_xcd", "graph", "trace_overlap", "batch_factor")}
+                           "trace_xcd", "trace_refill", "graph", "trace_overlap", "batch_factor")}
         # counting pass: exact node visits / triangle tests of one step
         cj = ctx.job(sid, scene.cam, W, H, params, base_seed=BASE_SEED)
         cj.set_counting(True)

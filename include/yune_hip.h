@@ -55,7 +55,7 @@ typedef struct yn_config {
                                  in the tree (0: 0.01; < 0: off)                    */
     int32_t leaf_max;         /* leaves of more triangles are split (1..10; 0: 2)   */
     int32_t lds_stack;        /* traversal stack entries per lane kept in LDS:
-                                 8, 16 or 32 (0: 8); deeper entries spill to HBM   */
+                                 8, 16 or 32 (0: 16); deeper entries spill to HBM  */
     int32_t shade_waves;      /* occupancy the shade kernel is compiled for: 2, 3, 4
                                  (0: 4)                                             */
     int32_t batch_factor;     /* path-tracing work per batch over paths in flight:
@@ -67,6 +67,11 @@ typedef struct yn_config {
                                  a HIP graph and replayed; 0: launched one by one  */
     int32_t trace_xcd;        /* ray-queue slices of a trace launch, one per XCD with
                                  its own work counter (1..16; 0: 1, one queue)     */
+    int32_t trace_refill;     /* trace lanes whose ray has ended take new rays once
+                                 this many lanes of the wave are idle (1..64);
+                                 -1: off, a wave runs its rays to the last one;
+                                 0: 48                                             */
+    int32_t pad;
 } yn_config;
 
 /* Parameters of udpt.cl / bdpt.cl as shipped: GI on, MIS off, no bounce
@@ -222,8 +227,8 @@ double yn_clock_ms(yn_ctx* ctx);                     /* ms since yn_create */
  * timed runs of the given view (frames per run) and keeps, knob by knob,
  * the fastest BVH width (the scenes' GPU layout is rebuilt), trace block
  * size, rays per queue fetch, LDS stack depth, shade occupancy, paths
- * per batch, batch_factor, trace_overlap, graph and trace_xcd in the context's
- * configuration.  Not while
+ * per batch, batch_factor, trace_overlap, graph, trace_xcd and trace_refill in
+ * the context's configuration.  Not while
  * served jobs are pending.  yn_config_get reads the configuration in effect. ---- */
 typedef struct yn_tune_result {
     int32_t bvh_width, trace_block, trace_chunk, lds_stack, shade_waves, n_tried;
@@ -233,6 +238,8 @@ typedef struct yn_tune_result {
     int32_t trace_overlap;    /* closest-hit and any-hit traces on two streams        */
     int32_t graph;            /* bounce loop replayed from a HIP graph                */
     int32_t trace_xcd;        /* ray-queue slices per trace launch                    */
+    int32_t trace_refill;     /* idle lanes before a trace wave takes new rays (-1 off) */
+    int32_t pad;
     double  mrays_per_s;      /* the chosen configuration on the tuning view */
 } yn_tune_result;
 int  yn_autotune(yn_ctx* ctx, int32_t scene_id, const yn_cam* cam, int32_t width, int32_t height,

@@ -58,12 +58,12 @@ int main(void) {
   S(yn_light_quad) O(yn_light_quad, ke) O(yn_light_quad, edge_w)
   S(yn_render_params) O(yn_render_params, fix_mis) O(yn_render_params, background)
   S(yn_config) O(yn_config, mem_budget) O(yn_config, bvh_width) O(yn_config, hoist_frac) O(yn_config, leaf_max) \
-    O(yn_config, lds_stack) O(yn_config, shade_waves) O(yn_config, batch_factor) O(yn_config, trace_overlap) O(yn_config, graph) O(yn_config, trace_xcd)
+    O(yn_config, lds_stack) O(yn_config, shade_waves) O(yn_config, batch_factor) O(yn_config, trace_overlap) O(yn_config, graph) O(yn_config, trace_xcd) O(yn_config, trace_refill)
   S(yn_host_scene) O(yn_host_scene, n_tri) O(yn_host_scene, mats) O(yn_host_scene, n_mat)
     O(yn_host_scene, light) O(yn_host_scene, cam)
   S(yn_scene_info) O(yn_scene_info, n_hoist) O(yn_scene_info, node_bytes) O(yn_scene_info, shade_bytes)
   S(yn_stats) O(yn_stats, iterations) O(yn_stats, ms_total) O(yn_stats, ms_other)
-  S(yn_tune_result) O(yn_tune_result, batch_paths) O(yn_tune_result, graph) O(yn_tune_result, trace_xcd) O(yn_tune_result, mrays_per_s)
+  S(yn_tune_result) O(yn_tune_result, batch_paths) O(yn_tune_result, graph) O(yn_tune_result, trace_xcd) O(yn_tune_result, trace_refill) O(yn_tune_result, mrays_per_s)
   S(yn_job_status) S(yn_camctl)
   return 0;
 }

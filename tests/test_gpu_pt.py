@@ -378,3 +378,40 @@ def test_trace_queue_slices_keep_images(mode):
         assert np.array_equal(imgs[0].view(np.int32), imgs[k].view(np.int32))
         for a, b in zip(traced[0], traced[k]):
             assert np.array_equal(np.asarray(a).view(np.int32), np.asarray(b).view(np.int32))
+
+
+@pytest.mark.parametrize("mode", [Y.YN_MODE_PT, Y.YN_MODE_BDPT])
+def test_trace_refill_keeps_images_and_counts(mode):
+    """Trace waves whose finished lanes take new rays (from one idle lane to
+    a whole wave) give the images of waves that run their rays to the end,
+    bit for bit, with two scenes in one batch (per-lane scene lookup), and
+    visit exactly the same nodes and triangles (counting mode); caller rays
+    (hoisted slots, brute force) trace the same too."""
+    scenes = [Y.gen_icosphere_room(3, 9, 2), Y.gen_cornell()]
+    kw = dict(bdpt_bounces=4) if mode else dict(mis=1, max_bounces=8, rr_threshold=2)
+    p = Y.params_reference(mode, **kw)
+    rays = random_rays(12000, 4, (-4.9, -4.9, -4.9), (4.9, 4.9, 4.9))
+    res = {}
+    for rf in (-1, 1, 16, 64):
+        with Y.Context(max_paths=1 << 14, trace_refill=rf) as c:
+            assert c.config()["trace_refill"] == rf
+            sids = [c.scene(s) for s in scenes]
+            jobs = [c.submit(sids[k], s.cam, 48, 40, p, 2, base_seed=31 + k) for k, s in enumerate(scenes)]
+            for _ in range(100):
+                if c.serve(4) == 0:
+                    break
+            imgs = [j.read() for j in jobs]
+            cj = c.job(sids[0], scenes[0].cam, 40, 32, p, base_seed=5)
+            cj.set_counting(True)
+            cj.render(1)
+            st = cj.stats()
+            tr = c.trace(sids[0], rays) + c.trace(c.scene(scenes[1], brute=True), rays[:4000])
+            res[rf] = (imgs, (st.rays_ext, st.nodes_ext, st.tris_ext, st.rays_shadow, st.nodes_shadow,
+                              st.tris_shadow), tr)
+    for rf in (1, 16, 64):
+        for a, b in zip(res[-1][0], res[rf][0]):
+            assert np.array_equal(a.view(np.int32), b.view(np.int32)), rf
+        assert res[rf][1] == res[-1][1], rf
+        for a, b in zip(res[-1][2], res[rf][2]):
+            assert np.array_equal(np.asarray(a).view(np.int32), np.asarray(b).view(np.int32)), rf
+    assert res[-1][1][1] > 0 and res[-1][1][4] > 0

@@ -158,9 +158,9 @@ class Context:
     def __init__(self, device: int = 0, max_paths: int = 0, bvh_width: int = 0, trace_block: int = 0,
                  trace_chunk: int = 0, trace_grid: int = 0, mem_budget: int = 0, hoist_frac: float = 0.0,
                  leaf_max: int = 0, lds_stack: int = 0, shade_waves: int = 0, batch_factor: int = 0,
-                 trace_overlap: int = 0, graph: int = 0, trace_xcd: int = 0):
+                 trace_overlap: int = 0, graph: int = 0, trace_xcd: int = 0, trace_refill: int = 0):
         cfg = Config(device, max_paths, mem_budget, bvh_width, trace_block, trace_chunk, trace_grid, hoist_frac,
-                     leaf_max, lds_stack, shade_waves, batch_factor, trace_overlap, graph, trace_xcd)
+                     leaf_max, lds_stack, shade_waves, batch_factor, trace_overlap, graph, trace_xcd, trace_refill)
         self._p = C.c_void_p()
         check(lib().yn_create(C.byref(cfg), C.byref(self._p)))
         self.cfg = cfg

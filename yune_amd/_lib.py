@@ -50,7 +50,8 @@ class Config(C.Structure):
                 ("bvh_width", C.c_int32), ("trace_block", C.c_int32), ("trace_chunk", C.c_int32),
                 ("trace_grid", C.c_int32), ("hoist_frac", C.c_float), ("leaf_max", C.c_int32),
                 ("lds_stack", C.c_int32), ("shade_waves", C.c_int32), ("batch_factor", C.c_int32),
-                ("trace_overlap", C.c_int32), ("graph", C.c_int32), ("trace_xcd", C.c_int32)]
+                ("trace_overlap", C.c_int32), ("graph", C.c_int32), ("trace_xcd", C.c_int32),
+                ("trace_refill", C.c_int32), ("pad", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -77,7 +78,8 @@ class TuneResult(C.Structure):
     _fields_ = [("bvh_width", C.c_int32), ("trace_block", C.c_int32), ("trace_chunk", C.c_int32),
                 ("lds_stack", C.c_int32), ("shade_waves", C.c_int32), ("n_tried", C.c_int32),
                 ("batch_paths", C.c_int64), ("batch_factor", C.c_int32), ("trace_overlap", C.c_int32),
-                ("graph", C.c_int32), ("trace_xcd", C.c_int32), ("mrays_per_s", C.c_double)]
+                ("graph", C.c_int32), ("trace_xcd", C.c_int32), ("trace_refill", C.c_int32), ("pad", C.c_int32),
+                ("mrays_per_s", C.c_double)]
 
 
 class JobStatus(C.Structure):

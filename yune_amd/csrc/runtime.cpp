@@ -449,6 +449,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         ta.hit_out = c->P.hit;
         ta.work = c->ctr->work[0];
         ta.n_slices = c->cfg.trace_xcd;
+        ta.refill = c->cfg.trace_refill > 0 ? c->cfg.trace_refill : 0;
         ta.zero0 = c->E[nxt].count;
         ta.zero1 = SQ[nxt].count;
         // shadow rays
@@ -935,7 +936,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.hoist_frac == 0.0f) k.hoist_frac = 0.01f;
     if (k.leaf_max <= 0) k.leaf_max = 2;
     if (k.leaf_max > 10) k.leaf_max = 10;
-    if (k.lds_stack == 0) k.lds_stack = 8;
+    if (k.lds_stack == 0) k.lds_stack = 16;
     if (k.shade_waves == 0) k.shade_waves = 4;
     if (k.lds_stack != 8 && k.lds_stack != 16 && k.lds_stack != 32)
         return fail(nullptr, YN_ERR_INVALID, "lds_stack must be 8, 16 or 32");
@@ -945,6 +946,8 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.trace_overlap != 0 && k.trace_overlap != 1) return fail(nullptr, YN_ERR_INVALID, "trace_overlap must be 0 or 1");
     if (k.graph != 0 && k.graph != 1) return fail(nullptr, YN_ERR_INVALID, "graph must be 0 or 1");
     if (k.trace_xcd == 0) k.trace_xcd = 1;
+    if (k.trace_refill == 0) k.trace_refill = 48;
+    if (k.trace_refill < -1 || k.trace_refill > 64) return fail(nullptr, YN_ERR_INVALID, "trace_refill must be -1 or 1..64");
     if (k.trace_xcd < 1 || k.trace_xcd > kMaxXcd) return fail(nullptr, YN_ERR_INVALID, "trace_xcd must be 1..16");
     if (k.bvh_width != 2 && k.bvh_width != 4) return fail(nullptr, YN_ERR_INVALID, "bvh_width must be 2 or 4");
     if (k.trace_block != 64 && k.trace_block != 128 && k.trace_block != 256)
@@ -1369,13 +1372,14 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         {0, {4, 2}},                            // BVH width
         {1, {256, 128, 64}},                    // trace block
         {2, {64, 32, 128}},                     // rays per wave per queue fetch
-        {3, {8, 16, 32}},                       // LDS stack entries
+        {3, {16, 8, 32}},                       // LDS stack entries
         {4, {2, 3, 4}},                         // shade occupancy
         {5, {cap, cap / 2, cap / 4}},           // paths per batch
         {6, {1, 2}},                            // work per batch over paths in flight (regeneration)
         {7, {0, 1}},                            // overlapping closest-hit and any-hit traces
         {8, {0, 1}},                            // bounce loop replayed from a graph
         {9, {1, 8}},                            // queue slices of the traces (one per XCD)
+        {10, {48, 32, -1}},                     // idle lanes before a trace wave takes new rays
     };
     auto get = [&](int kind) -> int64_t {
         switch (kind) {
@@ -1388,7 +1392,8 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         case 6: return c->cfg.batch_factor;
         case 7: return c->cfg.trace_overlap;
         case 8: return c->cfg.graph;
-        default: return c->cfg.trace_xcd;
+        case 9: return c->cfg.trace_xcd;
+        default: return c->cfg.trace_refill;
         }
     };
     auto set = [&](int kind, int64_t v) -> int {
@@ -1402,7 +1407,8 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         case 6: c->cfg.batch_factor = (int)v; return 0;
         case 7: c->cfg.trace_overlap = (int)v; return 0;
         case 8: c->cfg.graph = (int)v; return 0;
-        default: c->cfg.trace_xcd = (int)v; return 0;
+        case 9: c->cfg.trace_xcd = (int)v; return 0;
+        default: c->cfg.trace_refill = (int)v; return 0;
         }
     };
     double best = measure();
@@ -1433,6 +1439,7 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
     out->trace_overlap = c->cfg.trace_overlap;
     out->graph = c->cfg.graph;
     out->trace_xcd = c->cfg.trace_xcd;
+    out->trace_refill = c->cfg.trace_refill;
     out->mrays_per_s = best / 1e6;
     out->n_tried = tried;
     return 0;
@@ -1556,6 +1563,7 @@ int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri,
         ta.occ_out = any ? c->P.occ : nullptr;
         ta.work = c->ctr->work[0];
         ta.n_slices = c->cfg.trace_xcd;
+        ta.refill = c->cfg.trace_refill > 0 ? c->cfg.trace_refill : 0;
         int grid = c->n_cu * (2048 / c->cfg.trace_block);
         if (int rr = ensure_overflow(c, S.info.max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * c->cfg.trace_block))
             return rr;

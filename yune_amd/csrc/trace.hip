@@ -273,6 +273,88 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
     }
 }
 
+
+// One step of a ray's traversal, the body of traverse()'s loop: an interior
+// node (descending to its nearest child) or a leaf, then a pop when nothing
+// was descended into.  Returns true while the ray has nodes left; `node`,
+// the stack and the best hit carry over between steps.
+template <int W, bool ANY, int LST, int BLOCK, bool COUNT, bool EXACT>
+__device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, const float4* __restrict__ tri,
+                                          uint32_t& node, RayState& r, Stack<LST, BLOCK>& st, bool& got, Counts& c)
+{
+    if (!(node & kLeafBit)) {
+        if (COUNT) c.nodes++;
+        if (W == 2) {
+            const float4* n = nodes + 4 * (size_t)node;
+            float4 X = n[0], Y = n[1], Z = n[2];
+            int4 C = *reinterpret_cast<const int4*>(n + 3);
+            float e0, e1;
+            bool h0 = box_hit<EXACT>(X.x, Y.x, Z.x, X.z, Y.z, Z.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e0);
+            bool h1 = box_hit<EXACT>(X.y, Y.y, Z.y, X.w, Y.w, Z.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e1);
+            h0 = h0 && e0 <= r.cull && (uint32_t)C.x != kEmptyRef;
+            h1 = h1 && e1 <= r.cull && (uint32_t)C.y != kEmptyRef;
+            if (h0 && h1) {
+                uint32_t nearc = (uint32_t)C.x, farc = (uint32_t)C.y;
+                float fe = e1;
+                if (e1 < e0) { nearc = (uint32_t)C.y; farc = (uint32_t)C.x; fe = e0; }
+                if (!st.push(farc, fe)) { r.overflow = 1; return false; }   // cannot happen: depth bound
+                node = nearc;
+                return true;
+            } else if (h0) { node = (uint32_t)C.x; return true; }
+            else if (h1) { node = (uint32_t)C.y; return true; }
+        } else {
+            const float4* n = nodes + 8 * (size_t)node;
+            float4 LX = n[0], HX = n[1], LY = n[2], HY = n[3], LZ = n[4], HZ = n[5];
+            int4 C = *reinterpret_cast<const int4*>(n + 6);
+            float k0, k1, k2, k3;
+            bool h0 = box_hit<EXACT>(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k0);
+            bool h1 = box_hit<EXACT>(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k1);
+            bool h2 = box_hit<EXACT>(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k2);
+            bool h3 = box_hit<EXACT>(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k3);
+            uint32_t c0 = (uint32_t)C.x, c1 = (uint32_t)C.y, c2 = (uint32_t)C.z, c3 = (uint32_t)C.w;
+            // missed, culled or absent children sort last (key +inf)
+            k0 = (h0 && k0 <= r.cull && c0 != kEmptyRef) ? k0 : INFINITY;
+            k1 = (h1 && k1 <= r.cull && c1 != kEmptyRef) ? k1 : INFINITY;
+            k2 = (h2 && k2 <= r.cull && c2 != kEmptyRef) ? k2 : INFINITY;
+            k3 = (h3 && k3 <= r.cull && c3 != kEmptyRef) ? k3 : INFINITY;
+            // 4-input sorting network, ascending entry distance
+#define YN_CS(ka, ca, kb, cb)                                              \
+{                                                                      \
+    bool sw = kb < ka;                                                 \
+    float tk = sw ? kb : ka; kb = sw ? ka : kb; ka = tk;               \
+    uint32_t tc = sw ? cb : ca; cb = sw ? ca : cb; ca = tc;            \
+}
+            YN_CS(k0, c0, k1, c1);
+            YN_CS(k2, c2, k3, c3);
+            YN_CS(k0, c0, k2, c2);
+            YN_CS(k1, c1, k3, c3);
+            YN_CS(k1, c1, k2, c2);
+#undef YN_CS
+            if (k0 != INFINITY) {
+                bool ok = true;
+                if (k3 != INFINITY) ok &= st.push(c3, k3);
+                if (k2 != INFINITY) ok &= st.push(c2, k2);
+                if (k1 != INFINITY) ok &= st.push(c1, k1);
+                if (!ok) { r.overflow = 1; return false; }   // cannot happen: depth bound
+                node = c0;
+                return true;
+            }
+        }
+    } else {
+        got |= leaf_test<ANY, COUNT>(tri, node, r, c);
+        if (ANY && got) return false;
+    }
+    // pop; a deferred child whose entry now lies beyond the best hit is dropped
+    for (;;) {
+        if (st.sp == 0) return false;
+        float et;
+        uint32_t ref = st.pop(et);
+        if (et > r.cull) continue;
+        node = ref;
+        return true;
+    }
+}
+
 }  // namespace
 
 // Occupancy the kernel is compiled for, per SIMD: the LDS stack (6 B per
@@ -396,12 +478,163 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
     }
 }
 
+// Result of one ray into the path's slot.
+template <bool ANY>
+__device__ __forceinline__ void store_hit(const TraceArgs& a, int pidx, const RayState& r, bool got)
+{
+    if (r.overflow) atomicOr(&a.ctr->q[7], 1u);
+    if (ANY) {
+        a.occ_out[pidx] = got ? 1 : 0;
+        if (a.hit_out) a.hit_out[pidx] = make_float4(r.best, __int_as_float(got ? r.tri : -1), r.u, r.v);
+    } else {
+        a.hit_out[pidx] = make_float4(r.best, __int_as_float(r.tri), r.u, r.v);
+    }
+}
+
+// Traversal with lanes refilled.  k_trace runs each lane's ray to its end
+// while the wave waits for its slowest ray, so about a quarter of the vector
+// lanes do work.  Here the loop takes one traversal step per lane per
+// iteration (traverse()'s loop body), and once `a.refill` lanes of the wave
+// have finished, those lanes take new rays from the queue, one atomic for
+// all of them.  Each ray's own steps, tests and results are those of
+// k_trace.  The first fetch of every wave is static (rays wave*64 ..),
+// later ones come after the static range from the work counter.
+template <int W, bool ANY, int LST, int BLOCK, bool COUNT>
+__global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArgs a)
+{
+    __shared__ uint32_t stk_all[LST * BLOCK];
+    __shared__ uint16_t stkt_all[LST * BLOCK];
+    Stack<LST, BLOCK> st;
+    st.lref = stk_all + threadIdx.x;
+    st.lt = stkt_all + threadIdx.x;
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    st.gstride = gridDim.x * BLOCK;
+    st.gref = a.ovf_ref + gtid;
+    st.gt = a.ovf_t + gtid;
+    st.gcap = a.ovf_depth;
+    st.sp = 0;
+    const int lane = (int)__lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.zero0) { *a.zero0 = 0; if (a.zero1) *a.zero1 = 0; }
+    const uint32_t n = *a.q.count;
+    const uint32_t wave = gtid >> 6;
+    const uint32_t n_static = (gridDim.x * BLOCK) >> 6;
+    // a wave with every lane idle always fetches, so the queue drains
+    const int refill = a.refill < 1 ? 1 : (a.refill > 64 ? 64 : a.refill);
+    Counts c{0, 0};
+    unsigned nrays = 0;
+    RayState r{};
+    uint32_t node = 0;
+    int pidx = 0;
+    bool live = false, got = false, exact = false;
+    const float4* nodes = nullptr;
+    const float4* tri = nullptr;
+    bool more = true, first = true;
+    for (;;) {
+        const unsigned long long idle = __ballot(!live);
+        const int n_idle = __popcll(idle);
+        if (more && n_idle >= refill) {
+            uint32_t base;
+            if (first) {
+                base = wave * 64u;               // every lane is idle on the first fetch
+            } else {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(a.work, (uint32_t)n_idle);
+                base = __shfl(b, 0) + n_static * 64u;
+            }
+            first = false;
+            if (base + (uint32_t)n_idle >= n) more = false;
+            const uint32_t i = base + (uint32_t)__popcll(idle & below);
+            if (!live && i < n) {
+                float4 ot = a.q.o_t[i];
+                float4 dp = a.q.d_p[i];
+                pidx = __float_as_int(dp.w);
+                int sc = a.single_scene >= 0 ? a.single_scene : a.jobs[a.path_job[pidx >> a.id_shift]].scene;
+                const DevScene& S = a.scenes[sc];
+                r.ox = ot.x; r.oy = ot.y; r.oz = ot.z;
+                r.dx = dp.x; r.dy = dp.y; r.dz = dp.z;
+                r.ix = 1 / r.dx; r.iy = 1 / r.dy; r.iz = 1 / r.dz;   // dir_inv (udpt.cl:395)
+                r.best = ot.w;
+                r.cull = ot.w * 1.0001f;
+                r.tri = -1; r.u = 0.0f; r.v = 0.0f; r.overflow = 0;
+                st.sp = 0;
+                got = false;
+                ++nrays;
+                // only an infinite dir_inv can make the reference's NaN guard matter
+                exact = fabsf(r.ix) == INFINITY || fabsf(r.iy) == INFINITY || fabsf(r.iz) == INFINITY;
+                // a ray with a NaN component hits nothing (see k_trace)
+                live = ot.x == ot.x && ot.y == ot.y && ot.z == ot.z && ot.w == ot.w && dp.x == dp.x &&
+                       dp.y == dp.y && dp.z == dp.z;
+                nodes = S.nodes;
+                tri = S.tri_isect;
+                // the tests ahead of the tree, as in traverse(): every slot
+                // without a BVH; the hoisted slots; a root that is one leaf
+                if (live && S.brute) {
+                    for (int s = 0; s < S.n_slots; ++s) {
+                        if (COUNT) c.tris++;
+                        got |= tri_test(tri, s, r);
+                        if (ANY && got) break;
+                    }
+                    live = false;
+                }
+                if (live) {
+                    for (int s = 0; s < S.n_hoist; ++s) {
+                        float4 lo = S.hoist_box[2 * s], hi = S.hoist_box[2 * s + 1];
+                        float e;
+                        if (!box_hit(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e) ||
+                            e > r.cull)
+                            continue;
+                        if (COUNT) c.tris++;
+                        got |= tri_test(tri, s, r);
+                        if (ANY && got) break;
+                    }
+                    node = S.root_ref;
+                    if ((ANY && got) || node == kEmptyRef) {
+                        live = false;
+                    } else if (node & kLeafBit) {
+                        float e;
+                        if (box_hit(S.root_box[0], S.root_box[1], S.root_box[2], S.root_box[3], S.root_box[4],
+                                    S.root_box[5], r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e))
+                            got |= leaf_test<ANY, COUNT>(tri, node, r, c);
+                        live = false;
+                    }
+                }
+                if (!live) store_hit<ANY>(a, pidx, r, got);
+            }
+        }
+        if (__ballot(live) == 0) {
+            if (!more) break;
+            continue;
+        }
+        const bool wexact = __ballot(live && exact) != 0;
+        if (live) {
+            live = wexact ? trav_step<W, ANY, LST, BLOCK, COUNT, true>(nodes, tri, node, r, st, got, c)
+                          : trav_step<W, ANY, LST, BLOCK, COUNT, false>(nodes, tri, node, r, st, got, c);
+            if (!live) store_hit<ANY>(a, pidx, r, got);
+        }
+    }
+    unsigned long long tot = wave_sum_u64(nrays);
+    if (lane == 0 && tot) atomicAdd(ANY ? &a.ctr->rays_shadow : &a.ctr->rays_ext, tot);
+    if (COUNT) {
+        unsigned long long tn = wave_sum_u64(c.nodes), tt = wave_sum_u64(c.tris);
+        if (lane == 0) {
+            atomicAdd(ANY ? &a.ctr->nodes_shadow : &a.ctr->nodes_ext, tn);
+            atomicAdd(ANY ? &a.ctr->tris_shadow : &a.ctr->tris_ext, tt);
+        }
+    }
+}
+
 namespace {
 template <int W, bool ANY, int LST, int BL>
 void launch_b(const TraceArgs& a, int grid, bool count, hipStream_t s)
 {
-    if (count) hipLaunchKernelGGL((k_trace<W, ANY, LST, BL, true>), dim3(grid), dim3(BL), 0, s, a);
-    else hipLaunchKernelGGL((k_trace<W, ANY, LST, BL, false>), dim3(grid), dim3(BL), 0, s, a);
+    if (a.refill > 0) {
+        if (count) hipLaunchKernelGGL((k_trace_rf<W, ANY, LST, BL, true>), dim3(grid), dim3(BL), 0, s, a);
+        else hipLaunchKernelGGL((k_trace_rf<W, ANY, LST, BL, false>), dim3(grid), dim3(BL), 0, s, a);
+    } else {
+        if (count) hipLaunchKernelGGL((k_trace<W, ANY, LST, BL, true>), dim3(grid), dim3(BL), 0, s, a);
+        else hipLaunchKernelGGL((k_trace<W, ANY, LST, BL, false>), dim3(grid), dim3(BL), 0, s, a);
+    }
 }
 
 template <bool ANY, int LST>

@@ -19,6 +19,7 @@ struct TraceArgs {
     int32_t* occ_out;           // per path (any-hit)
     uint32_t* work;             // chunk counters, one per queue slice, zero at launch
     int32_t n_slices;           // queue slices (one per XCD, 1..kMaxXcd)
+    int32_t refill;             // > 0: lanes take new rays once this many of a wave are idle
     uint32_t* zero0;            // counters this launch clears (next queues)
     uint32_t* zero1;
     DevCounters* ctr;
