@@ -46,10 +46,8 @@ WORKLOADS = {
         desc="path tracing, 48 icospheres x 20480 tris + 12-tri room + 2-tri quad light ({n} tris), 1920x1080, "
              "{f} spp per step, max depth 8, RR from bounce 3, NEE+MIS"),
     "cornell": dict(
-        # short rays: eight queue slices (one work counter per XCD), waves run to their last ray, as
-        # yn_autotune picks for this scene
+        # short rays: eight queue slices (one work counter per XCD), as yn_autotune picks for this scene
         metric="Mrays/s Cornell box 512x512", W=512, H=512, frames=128, steps=8, mode=0, trace_xcd=8,
-        trace_refill=-1,
         params=dict(mis=1, max_bounces=8, rr_threshold=2),
         desc="path tracing, Cornell box ({n} tris), 512x512, {f} spp per step, max depth 8, RR from bounce 3, "
              "NEE+MIS"),

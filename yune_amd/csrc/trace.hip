@@ -517,10 +517,19 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
     const unsigned long long below = (1ull << lane) - 1ull;
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.zero0) { *a.zero0 = 0; if (a.zero1) *a.zero1 = 0; }
     const uint32_t n = *a.q.count;
-    const uint32_t wave = gtid >> 6;
-    const uint32_t n_static = (gridDim.x * BLOCK) >> 6;
     // a wave with every lane idle always fetches, so the queue drains
     const int refill = a.refill < 1 ? 1 : (a.refill > 64 ? 64 : a.refill);
+    // Queue slices as in k_trace, in rays: the wave's first fetch is its
+    // static 64 rays of its XCD's slice, later ones as many rays as idle
+    // lanes from the slice's counter, past the slice's static rays; a used-up
+    // slice sends the wave on to the next ones (never back).
+    const uint32_t ns = (uint32_t)a.n_slices;
+    const uint32_t G = gridDim.x, wpb = BLOCK >> 6;
+    const uint32_t wx = (blockIdx.x / ns) * wpb + (threadIdx.x >> 6);
+    auto waves_of = [&](uint32_t y) { return ((G - y + ns - 1) / ns) * wpb; };
+    auto slice_lo = [&](uint32_t y) { return (uint32_t)(((uint64_t)y * n) / ns); };
+    const bool all_static = (uint64_t)n <= (uint64_t)ns * ((G / ns) * wpb) * 64u;
+    uint32_t y = blockIdx.x % ns, moved = 0;
     Counts c{0, 0};
     unsigned nrays = 0;
     RayState r{};
@@ -534,18 +543,24 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
         const unsigned long long idle = __ballot(!live);
         const int n_idle = __popcll(idle);
         if (more && n_idle >= refill) {
-            uint32_t base;
-            if (first) {
-                base = wave * 64u;               // every lane is idle on the first fetch
-            } else {
-                uint32_t b = 0;
-                if (lane == 0) b = atomicAdd(a.work, (uint32_t)n_idle);
-                base = __shfl(b, 0) + n_static * 64u;
+            uint32_t base = 0, lim = 0;
+            for (;;) {
+                const uint32_t lo = slice_lo(y), hi = slice_lo(y + 1);
+                if (first) {
+                    base = lo + wx * 64u;            // every lane is idle on the first fetch
+                    first = false;
+                } else {
+                    uint32_t b = 0;
+                    if (lane == 0) b = atomicAdd(a.work + y, (uint32_t)n_idle);
+                    base = lo + waves_of(y) * 64u + __shfl(b, 0);
+                }
+                if (base < hi) { lim = hi; break; }
+                if (all_static || ++moved >= ns) { more = false; break; }
+                y = (y + 1) % ns;
             }
-            first = false;
-            if (base + (uint32_t)n_idle >= n) more = false;
+            if (all_static) more = false;            // nothing past the static rays
             const uint32_t i = base + (uint32_t)__popcll(idle & below);
-            if (!live && i < n) {
+            if (!live && i < lim) {
                 float4 ot = a.q.o_t[i];
                 float4 dp = a.q.d_p[i];
                 pidx = __float_as_int(dp.w);
