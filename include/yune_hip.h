@@ -66,7 +66,7 @@ typedef struct yn_config {
     int32_t graph;            /* 1: groups of bounce iterations are captured once in
                                  a HIP graph and replayed; 0: launched one by one  */
     int32_t trace_xcd;        /* ray-queue slices of a trace launch, one per XCD with
-                                 its own work counter (1..16; 0: 1, one queue)     */
+                                 its own work counter (1..16; 0: 8)                */
     int32_t trace_refill;     /* trace lanes whose ray has ended take new rays once
                                  this many lanes of the wave are idle (1..64);
                                  -1: off, a wave runs its rays to the last one;

@@ -421,23 +421,27 @@ emitted:
 
 __global__ __launch_bounds__(256) void k_shade_bdpt(BdptShadeArgs a, DevCapture cap)
 {
+    __shared__ BlockReserveLds<3, 256> rsv;
     const uint32_t n = *a.list_in_count;
-    if (blockIdx.x == 0 && (int)threadIdx.x < a.n_zero) a.zero[threadIdx.x] = 0;
-    const int lane = (int)__lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+    if (blockIdx.x == 0 && (int)threadIdx.x < a.n_zero) a.zero[threadIdx.x * kWorkStride] = 0;
     const int shift = a.B.shift;
-    for (uint32_t base = wave * 64; base < n; base += n_waves * 64) {
-        uint32_t i = base + (uint32_t)lane;
+    uint32_t* const ctr[3] = {a.ext_out.count, a.sh_out.count, a.list_out_count};
+    int par = 0;
+    // the block walks the list together: its appends are one atomic per queue
+    for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u, par ^= 1) {
+        uint32_t i = base + threadIdx.x;
         BOut o;
         int p = -1;
         if (i < n) {
             p = (int)a.list_in[i];
             o = advance_bdpt(a, cap, p);
         }
-        uint32_t e = wave_append(a.ext_out.count, o.ext);
-        if (o.ext) { a.ext_out.o_t[e] = o.o_t; a.ext_out.d_p[e] = o.d_p; }
-        uint32_t s = wave_reserve(a.sh_out.count, (uint32_t)__popc(o.emit));
+        const bool more = o.ext || o.emit != 0;
+        const uint32_t cnt[3] = {o.ext ? 1u : 0u, (uint32_t)__popc(o.emit), more ? 1u : 0u};
+        uint32_t at[3];
+        block_reserve<3, 256>(rsv, par, ctr, cnt, at);
+        if (o.ext) { a.ext_out.o_t[at[0]] = o.o_t; a.ext_out.d_p[at[0]] = o.d_p; }
+        uint32_t s = at[1];
         if (o.emit) {
             uint32_t m = o.emit;
             if (m & 1u) {
@@ -463,9 +467,7 @@ __global__ __launch_bounds__(256) void k_shade_bdpt(BdptShadeArgs a, DevCapture 
                 }
             }
         }
-        bool more = o.ext || o.emit != 0;
-        uint32_t l = wave_append(a.list_out_count, more);
-        if (more) a.list_out[l] = (uint32_t)p;
+        if (more) a.list_out[at[2]] = (uint32_t)p;
     }
 }
 

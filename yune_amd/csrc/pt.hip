@@ -373,14 +373,15 @@ steps_done:
 template <int WAVES>
 __global__ __launch_bounds__(256, WAVES) void k_shade_pt(ShadeArgs a, DevCapture cap)
 {
+    __shared__ BlockReserveLds<3, 256> rsv;
     const uint32_t n_ext = *a.ext_in.count, n_sh = *a.sh_in.count;
     const uint32_t total = n_ext + n_sh;
-    if (blockIdx.x == 0 && (int)threadIdx.x < a.n_zero) a.zero[threadIdx.x] = 0;
-    const int lane = (int)__lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t base = wave * 64; base < total; base += n_waves * 64) {
-        uint32_t i = base + (uint32_t)lane;
+    if (blockIdx.x == 0 && (int)threadIdx.x < a.n_zero) a.zero[threadIdx.x * kWorkStride] = 0;
+    uint32_t* const ctr[3] = {a.ext_out.count, a.sh_out.count, a.free_count};
+    int par = 0;
+    // the block walks the queues together: its appends are one atomic per queue
+    for (uint32_t base = blockIdx.x * 256u; base < total; base += gridDim.x * 256u, par ^= 1) {
+        uint32_t i = base + threadIdx.x;
         Out o;
         int pid = -1;
         if (i < total) {
@@ -389,14 +390,13 @@ __global__ __launch_bounds__(256, WAVES) void k_shade_pt(ShadeArgs a, DevCapture
             pid = __float_as_int((ext ? a.ext_in.d_p[k] : a.sh_in.d_p[k]).w);
             o = advance(a, cap, pid);
         }
-        uint32_t e = wave_append(a.ext_out.count, o.kind == 1);
-        if (o.kind == 1) { a.ext_out.o_t[e] = o.o_t; a.ext_out.d_p[e] = o.d_p; }
-        uint32_t s = wave_append(a.sh_out.count, o.kind == 2);
-        if (o.kind == 2) { a.sh_out.o_t[s] = o.o_t; a.sh_out.d_p[s] = o.d_p; }
-        if (a.free_list) {           // hand the slot back for regeneration
-            uint32_t f = wave_append(a.free_count, o.done);
-            if (o.done) a.free_list[f] = (uint32_t)pid;
-        }
+        const bool fr = a.free_list != nullptr && pid >= 0 && o.done;
+        const uint32_t n[3] = {o.kind == 1 ? 1u : 0u, o.kind == 2 ? 1u : 0u, fr ? 1u : 0u};
+        uint32_t at[3];
+        block_reserve<3, 256>(rsv, par, ctr, n, at);
+        if (o.kind == 1) { a.ext_out.o_t[at[0]] = o.o_t; a.ext_out.d_p[at[0]] = o.d_p; }
+        if (o.kind == 2) { a.sh_out.o_t[at[1]] = o.o_t; a.sh_out.d_p[at[1]] = o.d_p; }
+        if (fr) a.free_list[at[2]] = (uint32_t)pid;     // hand the slot back for regeneration
     }
 }
 

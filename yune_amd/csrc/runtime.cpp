@@ -447,7 +447,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         // extension rays
         ta.q = c->E[cur];
         ta.hit_out = c->P.hit;
-        ta.work = c->ctr->work[0];
+        ta.work = &c->ctr->work[0][0][0];
         ta.n_slices = c->cfg.trace_xcd;
         ta.refill = c->cfg.trace_refill > 0 ? c->cfg.trace_refill : 0;
         ta.zero0 = c->E[nxt].count;
@@ -459,7 +459,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         ts.hit_out = nullptr;
         ts.occ_out = bdpt ? c->B.occ : c->P.occ;
         ts.id_shift = bdpt ? c->B.shift : 0;
-        ts.work = c->ctr->work[1];
+        ts.work = &c->ctr->work[1][0][0];
         ts.zero0 = bdpt ? q + 8 + nxt : nullptr;    // the next path list
         ts.zero1 = nullptr;
         if (overlap) {          // the second half of the overflow area
@@ -478,7 +478,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             ba.list_out_count = q + 8 + nxt;
             ba.ext_out = c->E[nxt];
             ba.sh_out = c->ShB[nxt];
-            ba.zero = &c->ctr->work[0][0];
+            ba.zero = &c->ctr->work[0][0][0];
             ba.n_zero = 2 * kMaxXcd;
         } else {
             ShadeArgs& sa = A.sa;
@@ -489,7 +489,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             sa.sh_in = c->Sh[cur];
             sa.ext_out = c->E[nxt];
             sa.sh_out = c->Sh[nxt];
-            sa.zero = &c->ctr->work[0][0];
+            sa.zero = &c->ctr->work[0][0][0];
             sa.n_zero = 2 * kMaxXcd;
             sa.ctr = c->ctr;
             sa.free_list = regen ? c->free_list : nullptr;
@@ -569,7 +569,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             }
         }
         HIPCK(c, hipGetLastError());
-        HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, s));
+        HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, kCtrHostBytes, hipMemcpyDeviceToHost, s));
         HIPCK(c, hipStreamSynchronize(s));
         int nxt = it & 1;
         if (c->h_ctr->q[7]) return fail(c, YN_ERR_INVALID, "traversal stack overflow");
@@ -945,7 +945,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.batch_factor != 1 && k.batch_factor != 2) return fail(nullptr, YN_ERR_INVALID, "batch_factor must be 1 or 2");
     if (k.trace_overlap != 0 && k.trace_overlap != 1) return fail(nullptr, YN_ERR_INVALID, "trace_overlap must be 0 or 1");
     if (k.graph != 0 && k.graph != 1) return fail(nullptr, YN_ERR_INVALID, "graph must be 0 or 1");
-    if (k.trace_xcd == 0) k.trace_xcd = 1;
+    if (k.trace_xcd == 0) k.trace_xcd = 8;
     if (k.trace_refill == 0) k.trace_refill = 48;
     if (k.trace_refill < -1 || k.trace_refill > 64) return fail(nullptr, YN_ERR_INVALID, "trace_refill must be -1 or 1..64");
     if (k.trace_xcd < 1 || k.trace_xcd > kMaxXcd) return fail(nullptr, YN_ERR_INVALID, "trace_xcd must be 1..16");
@@ -1378,7 +1378,7 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
         {6, {1, 2}},                            // work per batch over paths in flight (regeneration)
         {7, {0, 1}},                            // overlapping closest-hit and any-hit traces
         {8, {0, 1}},                            // bounce loop replayed from a graph
-        {9, {1, 8}},                            // queue slices of the traces (one per XCD)
+        {9, {8, 16, 1}},                        // queue slices of the traces (one per XCD)
         {10, {48, 32, -1}},                     // idle lanes before a trace wave takes new rays
     };
     auto get = [&](int kind) -> int64_t {
@@ -1561,7 +1561,7 @@ int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri,
         ta.q = q;
         ta.hit_out = c->P.hit;
         ta.occ_out = any ? c->P.occ : nullptr;
-        ta.work = c->ctr->work[0];
+        ta.work = &c->ctr->work[0][0][0];
         ta.n_slices = c->cfg.trace_xcd;
         ta.refill = c->cfg.trace_refill > 0 ? c->cfg.trace_refill : 0;
         int grid = c->n_cu * (2048 / c->cfg.trace_block);
@@ -1575,7 +1575,7 @@ int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri,
         HIPCK(c, hipGetLastError());
         HIPCK(c, hipMemcpyAsync(hits.data(), c->P.hit, (size_t)m * 16, hipMemcpyDeviceToHost, s));
         HIPCK(c, hipMemcpyAsync(lights.data(), d_light, (size_t)m * 4, hipMemcpyDeviceToHost, s));
-        HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, s));
+        HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, kCtrHostBytes, hipMemcpyDeviceToHost, s));
         HIPCK(c, hipStreamSynchronize(s));
         if (c->h_ctr->q[7]) return fail(c, YN_ERR_INVALID, "traversal stack overflow");
         for (int k = 0; k < m; ++k) {

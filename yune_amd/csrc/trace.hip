@@ -415,7 +415,7 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
                 break;
             } else {
                 uint32_t k = 0;
-                if (lane == 0) k = atomicAdd(a.work + y, 1u);
+                if (lane == 0) k = atomicAdd(a.work + y * kWorkStride, 1u);
                 ci = c0 + waves_of(y) + __shfl(k, 0);
             }
             if (ci < c1) break;
@@ -551,7 +551,7 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
                     first = false;
                 } else {
                     uint32_t b = 0;
-                    if (lane == 0) b = atomicAdd(a.work + y, (uint32_t)n_idle);
+                    if (lane == 0) b = atomicAdd(a.work + y * kWorkStride, (uint32_t)n_idle);
                     base = lo + waves_of(y) * 64u + __shfl(b, 0);
                 }
                 if (base < hi) { lim = hi; break; }

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/yune_types.h"
@@ -97,9 +98,13 @@ struct BdptSoA {
 // Counters and stats in device memory.
 constexpr int kMaxXcd = 16;     // work counters per trace launch (one per queue slice)
 
+// Device-scope atomics on one cache line serialise; the work counters of
+// the queue slices are 256 bytes apart so that slices do not wait for each
+// other.  They come last, after the part the host reads back.
+constexpr int kWorkStride = 64;
+
 struct DevCounters {
     uint32_t q[12];             // queue counts, work counters (see runtime)
-    uint32_t work[2][kMaxXcd];  // chunk counters of the extend / shadow traces, per queue slice
     unsigned long long rays_ext;
     unsigned long long rays_shadow;
     unsigned long long nodes_ext;     // counting mode only: interior nodes visited
@@ -107,6 +112,8 @@ struct DevCounters {
     unsigned long long nodes_shadow;
     unsigned long long tris_shadow;
     unsigned long long pad[2];
+    uint32_t work[2][kMaxXcd][kWorkStride];   // chunk counters of the extend / shadow traces, per queue slice
 };
+constexpr size_t kCtrHostBytes = offsetof(DevCounters, work);   // what the host reads back
 
 }  // namespace yune

@@ -131,6 +131,58 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, uint32_t n)
     return base + x - n;
 }
 
+// Block-aggregated reservation in NQ queues at once: thread t asks for n[q]
+// consecutive slots of queue q and gets the first of them in first[q]
+// (threads in order).  One atomic per queue and block call, made by threads
+// 0..NQ-1 side by side: device-scope atomics on one counter serialise, and
+// with one per wave the shade passes spent most of their time waiting for
+// them.  Every thread of the block makes the same number of calls (there
+// are two barriers inside); `par` alternates between calls, so a call's
+// totals stay intact while a slower wave still reads the previous call's.
+// A null counter may be passed for a queue whose counts are all zero.
+template <int NQ, int BLOCK>
+struct BlockReserveLds {
+    uint32_t tot[2][BLOCK / 64][NQ];
+    uint32_t base[2][NQ];
+};
+
+template <int NQ, int BLOCK>
+__device__ __forceinline__ void block_reserve(BlockReserveLds<NQ, BLOCK>& sm, int par, uint32_t* const (&ctr)[NQ],
+                                              const uint32_t (&n)[NQ], uint32_t (&first)[NQ])
+{
+    const int lane = (int)__lane_id();
+    const int w = (int)(threadIdx.x >> 6);
+    uint32_t x[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        x[q] = n[q];
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x[q], o);
+            if (lane >= o) x[q] += y;
+        }
+        if (lane == 63) sm.tot[par][w][q] = x[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)NQ) {
+        const int q = (int)threadIdx.x;
+        uint32_t t = 0;
+        for (int v = 0; v < BLOCK / 64; ++v) t += sm.tot[par][v][q];
+        uint32_t* c = ctr[0];
+#pragma unroll
+        for (int k = 1; k < NQ; ++k)
+            if (q == k) c = ctr[k];
+        sm.base[par][q] = t ? atomicAdd(c, t) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        uint32_t b = sm.base[par][q];
+        for (int v = 0; v < w; ++v) b += sm.tot[par][v][q];
+        first[q] = b + x[q] - n[q];
+    }
+}
+
 // Wave-aggregated append: one atomic per wave, lanes get consecutive slots
 // in lane order.  Every active lane of the wave must call it.
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
