@@ -267,6 +267,7 @@ def main():
     ap.add_argument("--trace-xcd", type=int, default=0, help="ray-queue slices per trace launch (1: no slicing)")
     ap.add_argument("--trace-refill", type=int, default=0, help="idle lanes before a trace wave takes new rays (-1: off)")
     ap.add_argument("--leaf-max", type=int, default=0, help="triangles per BVH leaf (0: library default)")
+    ap.add_argument("--bvh-build", type=int, default=0, help="GPU tree: 1 own SAH build, 2 the reference's tree")
     ap.add_argument("--hoist-frac", type=float, default=0.0, help="box-area fraction above which triangles are hoisted")
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC counters")
     args = ap.parse_args()
@@ -302,12 +303,12 @@ def main():
                    max_paths=args.max_paths, batch_factor=args.batch_factor,
                    trace_overlap=args.trace_overlap, graph=args.graph,
                    trace_xcd=args.trace_xcd or wl.get("trace_xcd", 0),
-                   trace_refill=args.trace_refill or wl.get("trace_refill", 0)) as ctx:
+                   trace_refill=args.trace_refill or wl.get("trace_refill", 0), bvh_build=args.bvh_build) as ctx:
         sid = ctx.scene(scene, nodes)
         info = ctx.scene_info(sid)
         launch = {k: v for k, v in ctx.config().items()
                   if k in ("max_paths", "lds_stack", "shade_waves", "trace_block", "trace_chunk", "leaf_max",
-                           "trace_xcd", "trace_refill", "graph", "trace_overlap", "batch_factor")}
+                           "trace_xcd", "trace_refill", "graph", "trace_overlap", "batch_factor", "bvh_build")}
         # counting pass: exact node visits / triangle tests of one step
         cj = ctx.job(sid, scene.cam, W, H, params, base_seed=BASE_SEED)
         cj.set_counting(True)

@@ -53,7 +53,8 @@ typedef struct yn_config {
     float   hoist_frac;       /* triangles whose box area exceeds this fraction of the
                                  root box's are tested by every ray instead of living
                                  in the tree (0: 0.01; < 0: off)                    */
-    int32_t leaf_max;         /* leaves of more triangles are split (1..10; 0: 2)   */
+    int32_t leaf_max;         /* triangles per leaf at most (1..10; 0: 1 for the
+                                 library's own tree, 2 for the caller's)           */
     int32_t lds_stack;        /* traversal stack entries per lane kept in LDS:
                                  8, 16 or 32 (0: 16); deeper entries spill to HBM  */
     int32_t shade_waves;      /* occupancy the shade kernel is compiled for: 2, 3, 4
@@ -71,7 +72,10 @@ typedef struct yn_config {
                                  this many lanes of the wave are idle (1..64);
                                  -1: off, a wave runs its rays to the last one;
                                  0: 48                                             */
-    int32_t pad;
+    int32_t bvh_build;        /* GPU tree of a scene uploaded with a BVH: 1: the
+                                 library's own binned SAH build over the triangles
+                                 (3 axes, leaves of leaf_max); 2: the caller's
+                                 tree (the reference's), re-laid out; 0: 1         */
 } yn_config;
 
 /* Parameters of udpt.cl / bdpt.cl as shipped: GI on, MIS off, no bounce
@@ -90,6 +94,15 @@ int  yn_device_info(yn_ctx* ctx, char* name, int name_len, int32_t* n_cu, size_t
  * array.  *nodes is allocated by the library; release with yn_free. */
 int  yn_bvh_build(const yn_triangle* tris, int32_t n_tri, int32_t bins,
                   yn_bvh_node** nodes, int32_t* n_nodes);
+/* No counterpart in the reference: the tree a context with bvh_build = 1
+ * lays out on the GPU (before hoisting), the library's own binned SAH
+ * build over all three axes with leaves of at most leaf_max (1..10)
+ * triangles, in the reference's node format (BVHNodeGPU,
+ * include/CL_headers.h:85-91; children at child_idx and child_idx+1), so
+ * that a traversal of the reference's kind can walk it.  Release with
+ * yn_free. */
+int  yn_bvh_build_gpu(const yn_triangle* tris, int32_t n_tri, int32_t leaf_max,
+                      yn_bvh_node** nodes, int32_t* n_nodes);
 void yn_free(void* p);
 
 typedef struct yn_host_scene {

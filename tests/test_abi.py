@@ -58,7 +58,7 @@ int main(void) {
   S(yn_light_quad) O(yn_light_quad, ke) O(yn_light_quad, edge_w)
   S(yn_render_params) O(yn_render_params, fix_mis) O(yn_render_params, background)
   S(yn_config) O(yn_config, mem_budget) O(yn_config, bvh_width) O(yn_config, hoist_frac) O(yn_config, leaf_max) \
-    O(yn_config, lds_stack) O(yn_config, shade_waves) O(yn_config, batch_factor) O(yn_config, trace_overlap) O(yn_config, graph) O(yn_config, trace_xcd) O(yn_config, trace_refill)
+    O(yn_config, lds_stack) O(yn_config, shade_waves) O(yn_config, batch_factor) O(yn_config, trace_overlap) O(yn_config, graph) O(yn_config, trace_xcd) O(yn_config, trace_refill) O(yn_config, bvh_build)
   S(yn_host_scene) O(yn_host_scene, n_tri) O(yn_host_scene, mats) O(yn_host_scene, n_mat)
     O(yn_host_scene, light) O(yn_host_scene, cam)
   S(yn_scene_info) O(yn_scene_info, n_hoist) O(yn_scene_info, node_bytes) O(yn_scene_info, shade_bytes)
@@ -146,6 +146,8 @@ def test_host_entry_points_validate_arguments():
     assert lib.yn_bvh_build(None, -1, 20, C.byref(nodes), C.byref(n)) == -2
     assert lib.yn_bvh_build(None, 5, 20, C.byref(nodes), C.byref(n)) == -2
     assert lib.yn_bvh_build(None, 0, 20, None, C.byref(n)) == -2
+    assert lib.yn_bvh_build_gpu(None, 5, 2, C.byref(nodes), C.byref(n)) == -2
+    assert lib.yn_bvh_build_gpu(None, 0, 11, C.byref(nodes), C.byref(n)) == -2
     assert lib.yn_gen_icosphere_room(1, -1, 3, C.byref(L.HostScene())) == -2
     assert lib.yn_gen_icosphere_room(1, 2, 9, C.byref(L.HostScene())) == -2
     assert lib.yn_load_obj(None, C.byref(L.HostScene())) == -2

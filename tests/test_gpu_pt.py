@@ -63,10 +63,13 @@ def tri_t(s, tri_id, ray):
     return got, L.value
 
 
+@pytest.mark.parametrize("build", [1, 2])
 @pytest.mark.parametrize("hoist,leaf_max", [(0.01, 4), (-1.0, 10), (0.01, 1)])
 @pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("scene", ["cornell", "ico"])
-def test_trace_matches_oracle(width, scene, hoist, leaf_max):
+def test_trace_matches_oracle(width, scene, hoist, leaf_max, build):
+    """The GPU tree (the library's own, build 1, or the reference's re-laid
+    out, build 2) against the oracle walking the reference's tree."""
     if scene == "cornell":
         s = Y.gen_cornell()
         lo, hi = (-0.99, -0.99, -3.99), (0.99, 0.99, -2.01)
@@ -75,7 +78,7 @@ def test_trace_matches_oracle(width, scene, hoist, leaf_max):
         lo, hi = (-4.9, -4.9, -4.9), (4.9, 4.9, 4.9)
     nodes = s.bvh()
     rays = random_rays(20000, 5, lo, hi)
-    with Y.Context(bvh_width=width, max_paths=1 << 16, hoist_frac=hoist, leaf_max=leaf_max) as c:
+    with Y.Context(bvh_width=width, max_paths=1 << 16, hoist_frac=hoist, leaf_max=leaf_max, bvh_build=build) as c:
         sid = c.scene(s, nodes)
         assert (c.scene_info(sid).n_hoist > 0) == (hoist > 0)
         gtri, glight, gt = c.trace(sid, rays)
@@ -260,7 +263,7 @@ def test_deep_tree_uses_stack_overflow_area(width):
     rays[:, 0] = np.linspace(-0.01, 0.01, 64)
     rays[:, 3] = np.inf
     rays[:, 6] = -1.0
-    with Y.Context(bvh_width=width, max_paths=1 << 12, hoist_frac=-1, leaf_max=10) as c:
+    with Y.Context(bvh_width=width, max_paths=1 << 12, hoist_frac=-1, leaf_max=10, bvh_build=2) as c:
         sid = c.scene(s, s.nodes)
         assert c.scene_info(sid).max_depth > 16
         gtri, glight, gt = c.trace(sid, rays)

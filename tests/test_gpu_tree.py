@@ -1,0 +1,97 @@
+"""The library's own SAH tree (yn_bvh_build_gpu, laid out by contexts with
+bvh_build = 1) against the reference's tree, on the CPU.
+
+Checked here: it is a valid tree in the reference's node format (every
+triangle in exactly one leaf, leaves of at most leaf_max triangles,
+children after their parent, every box containing what lies below it); the
+oracle's traversal of it finds the same closest hits as on the reference's
+tree (same t bit for bit; a different triangle only at a verified tie) and
+the same shadow-ray answers; and it is the better tree by the SAH measure
+(summed interior-box area, the expected node visits of a random ray).
+"""
+import numpy as np
+import pytest
+
+import yune_amd as Y
+from oracle import pyoracle as O
+
+
+def tri_boxes(tris):
+    v = np.stack([tris["v1"][:, :3], tris["v2"][:, :3], tris["v3"][:, :3]], 1)
+    lo, hi = v.min(1), v.max(1)
+    hi = np.where(hi - lo == 0, hi + np.float32(0.2), hi)     # TriangleCPU.cpp:63-67
+    return lo, hi
+
+
+def check_tree(tris, nodes, leaf_max):
+    lo, hi = tri_boxes(tris)
+    seen = np.zeros(len(tris), np.int32)
+    for j, n in enumerate(nodes):
+        c, k = int(n["child_idx"]), int(n["vert_len"])
+        if c == -1:
+            assert 1 <= k <= leaf_max
+            ids = n["vert_list"][:k]
+            seen[ids] += 1
+            assert (n["p_min"][:3] <= lo[ids]).all() and (n["p_max"][:3] >= hi[ids]).all()
+        elif c > 0:
+            assert c > j and c + 1 < len(nodes)
+            for ch in nodes[c:c + 2]:
+                assert (n["p_min"][:3] <= ch["p_min"][:3]).all() and (n["p_max"][:3] >= ch["p_max"][:3]).all()
+        else:
+            assert j == 0 and len(nodes) == 1 and len(tris) == 0
+    assert (seen == 1).all(), "every triangle in exactly one leaf"
+
+
+def interior_area(nodes):
+    inner = nodes[nodes["child_idx"] > 0]
+    d = (inner["p_max"][:, :3] - inner["p_min"][:, :3]).astype(np.float64)
+    return float((d[:, 0] * d[:, 1] + d[:, 0] * d[:, 2] + d[:, 1] * d[:, 2]).sum())
+
+
+@pytest.mark.parametrize("leaf_max", [1, 2, 10])
+@pytest.mark.parametrize("scene", ["cornell", "ico"])
+def test_tree_is_valid(scene, leaf_max):
+    s = Y.gen_cornell() if scene == "cornell" else Y.gen_icosphere_room(5, 4, 2)
+    check_tree(s.tris, Y.build_gpu_bvh(s.tris, leaf_max), leaf_max)
+
+
+def test_empty_and_single_triangle():
+    s = Y.gen_cornell()
+    assert len(Y.build_gpu_bvh(s.tris[:0])) == 1
+    one = Y.build_gpu_bvh(s.tris[:1])
+    assert len(one) == 1 and one[0]["child_idx"] == -1 and one[0]["vert_len"] == 1
+
+
+def test_deterministic():
+    s = Y.gen_icosphere_room(7, 3, 2)
+    a, b = Y.build_gpu_bvh(s.tris), Y.build_gpu_bvh(s.tris)
+    assert a.tobytes() == b.tobytes()
+
+
+def test_same_hits_as_the_reference_tree():
+    s = Y.gen_icosphere_room(11, 6, 3)
+    rng = np.random.default_rng(3)
+    n = 6000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = rng.uniform(-4.9, 4.9, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    shadow = rng.uniform(size=n) < 0.3
+    rays[:, 3] = np.where(shadow, rng.uniform(0.05, 6.0, size=n), np.inf)
+    rays[:, 7] = shadow
+    a = O.OScene(s.tris, s.mats, s.bvh(), s.light, heap_size=0).trace(rays)
+    b = O.OScene(s.tris, s.mats, Y.build_gpu_bvh(s.tris), s.light, heap_size=0).trace(rays)
+    closest = ~shadow
+    hit = closest & (a[0] >= 0)
+    assert np.array_equal(a[0] >= 0, b[0] >= 0) and np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2][hit].view(np.int32), b[2][hit].view(np.int32)), "closest t differs"
+    ties = int((hit & (a[0] != b[0])).sum())       # equal t on two triangles, resolved by test order
+    assert ties <= 0.005 * hit.sum()
+    occ = lambda r: (r[0] >= 0) | (r[1] >= 0)     # noqa: E731
+    assert np.array_equal(occ(a)[shadow], occ(b)[shadow])
+
+
+def test_lower_sah_cost_than_the_reference_tree():
+    s = Y.gen_icosphere_room(2, 8, 3)
+    ours, ref = interior_area(Y.build_gpu_bvh(s.tris, 10)), interior_area(s.bvh())
+    assert ours < 0.8 * ref, (ours, ref)

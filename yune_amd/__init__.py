@@ -47,6 +47,20 @@ def params_reference(mode: int = YN_MODE_PT, **over) -> RenderParams:
     return p
 
 
+def build_gpu_bvh(tris: np.ndarray, leaf_max: int = 2) -> np.ndarray:
+    """The library's own SAH tree (what a context with bvh_build=1 lays out),
+    in the reference's node format."""
+    tris = np.ascontiguousarray(tris, dtype=TRI_DTYPE)
+    out = C.c_void_p()
+    n = C.c_int32()
+    check(lib().yn_bvh_build_gpu(ptr(tris), len(tris), leaf_max, C.byref(out), C.byref(n)))
+    try:
+        buf = (C.c_char * (n.value * NODE_DTYPE.itemsize)).from_address(out.value)
+        return np.frombuffer(bytes(buf), dtype=NODE_DTYPE).copy()
+    finally:
+        lib().yn_free(out)
+
+
 def build_bvh(tris: np.ndarray, bins: int = 20) -> np.ndarray:
     tris = np.ascontiguousarray(tris, dtype=TRI_DTYPE)
     out = C.c_void_p()
@@ -158,9 +172,11 @@ class Context:
     def __init__(self, device: int = 0, max_paths: int = 0, bvh_width: int = 0, trace_block: int = 0,
                  trace_chunk: int = 0, trace_grid: int = 0, mem_budget: int = 0, hoist_frac: float = 0.0,
                  leaf_max: int = 0, lds_stack: int = 0, shade_waves: int = 0, batch_factor: int = 0,
-                 trace_overlap: int = 0, graph: int = 0, trace_xcd: int = 0, trace_refill: int = 0):
+                 trace_overlap: int = 0, graph: int = 0, trace_xcd: int = 0, trace_refill: int = 0,
+                 bvh_build: int = 0):
         cfg = Config(device, max_paths, mem_budget, bvh_width, trace_block, trace_chunk, trace_grid, hoist_frac,
-                     leaf_max, lds_stack, shade_waves, batch_factor, trace_overlap, graph, trace_xcd, trace_refill)
+                     leaf_max, lds_stack, shade_waves, batch_factor, trace_overlap, graph, trace_xcd, trace_refill,
+                     bvh_build)
         self._p = C.c_void_p()
         check(lib().yn_create(C.byref(cfg), C.byref(self._p)))
         self.cfg = cfg

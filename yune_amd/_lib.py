@@ -51,7 +51,7 @@ class Config(C.Structure):
                 ("trace_grid", C.c_int32), ("hoist_frac", C.c_float), ("leaf_max", C.c_int32),
                 ("lds_stack", C.c_int32), ("shade_waves", C.c_int32), ("batch_factor", C.c_int32),
                 ("trace_overlap", C.c_int32), ("graph", C.c_int32), ("trace_xcd", C.c_int32),
-                ("trace_refill", C.c_int32), ("pad", C.c_int32)]
+                ("trace_refill", C.c_int32), ("bvh_build", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -114,6 +114,7 @@ SIGNATURES = {
     "yn_last_error": (C.c_char_p, [P]),
     "yn_device_info": (I32, [P, C.c_char_p, C.c_int, PI32, C.POINTER(C.c_size_t)]),
     "yn_bvh_build": (I32, [P, I32, I32, C.POINTER(P), PI32]),
+    "yn_bvh_build_gpu": (I32, [P, I32, I32, C.POINTER(P), PI32]),
     "yn_free": (None, [P]),
     "yn_load_obj": (I32, [C.c_char_p, C.POINTER(HostScene)]),
     "yn_gen_cornell": (I32, [C.POINTER(HostScene)]),

@@ -44,6 +44,20 @@ int yn_bvh_build(const yn_triangle* tris, int32_t n_tri, int32_t bins, yn_bvh_no
     return 0;
 }
 
+int yn_bvh_build_gpu(const yn_triangle* tris, int32_t n_tri, int32_t leaf_max, yn_bvh_node** nodes, int32_t* n_nodes)
+{
+    if (!nodes || !n_nodes || n_tri < 0 || (n_tri > 0 && !tris) || leaf_max < 1 || leaf_max > 10) {
+        set_global_error("bad arguments"); return YN_ERR_INVALID;
+    }
+    std::vector<yn_bvh_node> out;
+    int n = build_gpu_bvh(tris, n_tri, leaf_max, out);
+    *nodes = (yn_bvh_node*)std::malloc(sizeof(yn_bvh_node) * std::max<size_t>(1, out.size()));
+    if (!*nodes) return YN_ERR_NO_MEMORY;
+    if (!out.empty()) std::memcpy(*nodes, out.data(), sizeof(yn_bvh_node) * out.size());
+    *n_nodes = n;
+    return 0;
+}
+
 void yn_free(void* p) { std::free(p); }
 
 int yn_load_obj(const char* path, yn_host_scene* out)

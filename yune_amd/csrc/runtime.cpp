@@ -366,15 +366,15 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     HIPCK(c, hipMemsetAsync(c->ctr, 0, sizeof(DevCounters), s));
 
     const DevScene* scenes = c->d_scenes;
-    uint32_t* q = c->ctr->q;
-    // counters: q[0],q[1] extension queue counts, q[2],q[3] shadow queue
+    auto q = [&](int k) { return &c->ctr->q[k][0]; };
+    // counters (q(k) = q[k][0]): q[0],q[1] extension queue counts, q[2],q[3] shadow queue
     // counts, q[7] stack overflow flag, q[8],q[9] path-list counts
     // (bidirectional mode), q[10] free slots (regeneration); work[0], work[1]
     // the chunk counters of the extend / shadow traces, per queue slice.
     for (int k = 0; k < 2; ++k) {
-        c->E[k].count = q + k;
-        c->Sh[k].count = q + 2 + k;
-        c->ShB[k].count = q + 2 + k;
+        c->E[k].count = q(k);
+        c->Sh[k].count = q(2 + k);
+        c->ShB[k].count = q(2 + k);
     }
     RayQueue* SQ = bdpt ? c->ShB : c->Sh;
 
@@ -389,7 +389,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     if (bdpt) {
         ga.B = c->B;
         ga.list_out = c->lst[0];
-        ga.list_count = q + 8;
+        ga.list_count = q(8);
         launch_generate_bdpt(ga, cap, s);
     } else {
         launch_generate(ga, cap, s);
@@ -460,7 +460,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         ts.occ_out = bdpt ? c->B.occ : c->P.occ;
         ts.id_shift = bdpt ? c->B.shift : 0;
         ts.work = &c->ctr->work[1][0][0];
-        ts.zero0 = bdpt ? q + 8 + nxt : nullptr;    // the next path list
+        ts.zero0 = bdpt ? q(8 + nxt) : nullptr;    // the next path list
         ts.zero1 = nullptr;
         if (overlap) {          // the second half of the overflow area
             ts.ovf_ref = c->ovf_ref + (size_t)c->ovf_depth * grid * block;
@@ -473,9 +473,9 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             ba.P = c->P;
             ba.B = c->B;
             ba.list_in = c->lst[cur];
-            ba.list_in_count = q + 8 + cur;
+            ba.list_in_count = q(8 + cur);
             ba.list_out = c->lst[nxt];
-            ba.list_out_count = q + 8 + nxt;
+            ba.list_out_count = q(8 + nxt);
             ba.ext_out = c->E[nxt];
             ba.sh_out = c->ShB[nxt];
             ba.zero = &c->ctr->work[0][0][0];
@@ -493,7 +493,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
             sa.n_zero = 2 * kMaxXcd;
             sa.ctr = c->ctr;
             sa.free_list = regen ? c->free_list : nullptr;
-            sa.free_count = q + 10;
+            sa.free_count = q(10);
         }
     };
     if (max_depth > trace_lds_stack(c->cfg.lds_stack) + c->ovf_depth)
@@ -572,14 +572,14 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, kCtrHostBytes, hipMemcpyDeviceToHost, s));
         HIPCK(c, hipStreamSynchronize(s));
         int nxt = it & 1;
-        if (c->h_ctr->q[7]) return fail(c, YN_ERR_INVALID, "traversal stack overflow");
+        if (c->h_ctr->q[7][0]) return fail(c, YN_ERR_INVALID, "traversal stack overflow");
         if (it > 200000) return fail(c, YN_ERR_HIP, "wavefront loop did not drain");
-        bool empty = bdpt ? c->h_ctr->q[8 + nxt] == 0 : (c->h_ctr->q[nxt] == 0 && c->h_ctr->q[2 + nxt] == 0);
+        bool empty = bdpt ? c->h_ctr->q[8 + nxt][0] == 0 : (c->h_ctr->q[nxt][0] == 0 && c->h_ctr->q[2 + nxt][0] == 0);
         if (regen && issued < total) {
             // refill the slots freed so far once there are enough of them (or
             // the wavefront has drained), appending to the queue the next
             // iteration reads
-            const uint32_t f = c->h_ctr->q[10];
+            const uint32_t f = c->h_ctr->q[10][0];
             const int64_t need = total - issued;
             if (f > 0 && ((int64_t)f >= std::min<int64_t>(need, std::max<int64_t>(C / 8, 1)) || empty)) {
                 // slots come back in the order paths ended; sorted, the new
@@ -595,7 +595,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
                 gr.n_paths = (int32_t)std::min<int64_t>((int64_t)f, need);
                 gr.free_list = c->free_list;
                 gr.free_top = f;
-                gr.free_count = q + 10;
+                gr.free_count = q(10);
                 gr.ext_out = c->E[nxt];
                 launch_generate(gr, cap, s);
                 issued += gr.n_paths;
@@ -834,6 +834,7 @@ int build_scene(yn_ctx* c, int sid, int width)
     lo.width = width;
     lo.hoist_frac = c->cfg.hoist_frac;
     lo.leaf_max = c->cfg.leaf_max;
+    lo.rebuild = c->cfg.bvh_build == 1;
     int r = make_layout(S.h_tris.data(), (int32_t)S.h_tris.size(), (int32_t)S.h_mats.size(), S.h_nodes.data(),
                         (int32_t)S.h_nodes.size(), lo, L, err);
     if (r) return fail(c, r, err);
@@ -934,7 +935,9 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.trace_block == 0) k.trace_block = 256;
     if (k.trace_chunk <= 0) k.trace_chunk = 64;
     if (k.hoist_frac == 0.0f) k.hoist_frac = 0.01f;
-    if (k.leaf_max <= 0) k.leaf_max = 2;
+    if (k.bvh_build == 0) k.bvh_build = 1;
+    if (k.bvh_build != 1 && k.bvh_build != 2) return fail(nullptr, YN_ERR_INVALID, "bvh_build must be 1 or 2");
+    if (k.leaf_max <= 0) k.leaf_max = k.bvh_build == 1 ? 1 : 2;
     if (k.leaf_max > 10) k.leaf_max = 10;
     if (k.lds_stack == 0) k.lds_stack = 16;
     if (k.shade_waves == 0) k.shade_waves = 4;
@@ -1549,7 +1552,7 @@ int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri,
         HIPCK(c, hipMemcpyAsync(d_rays, sub.data(), sub.size() * 4, hipMemcpyHostToDevice, s));
         HIPCK(c, hipMemsetAsync(c->ctr, 0, sizeof(DevCounters), s));
         RayQueue q = c->E[0];
-        q.count = c->ctr->q;
+        q.count = &c->ctr->q[0][0];
         launch_prepare(c->d_scenes + sid, d_rays, m, q, d_light, s);
         TraceArgs ta{};
         ta.scenes = c->d_scenes;
@@ -1577,7 +1580,7 @@ int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri,
         HIPCK(c, hipMemcpyAsync(lights.data(), d_light, (size_t)m * 4, hipMemcpyDeviceToHost, s));
         HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, kCtrHostBytes, hipMemcpyDeviceToHost, s));
         HIPCK(c, hipStreamSynchronize(s));
-        if (c->h_ctr->q[7]) return fail(c, YN_ERR_INVALID, "traversal stack overflow");
+        if (c->h_ctr->q[7][0]) return fail(c, YN_ERR_INVALID, "traversal stack overflow");
         for (int k = 0; k < m; ++k) {
             int i = idx[any][k];
             int tr;

@@ -457,7 +457,7 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
                 got = __ballot(inf_inv) ? traverse<W, ANY, LST, BLOCK, COUNT, true>(S, r, st, c)
                                         : traverse<W, ANY, LST, BLOCK, COUNT, false>(S, r, st, c);
             ++nrays;
-            if (r.overflow) atomicOr(&a.ctr->q[7], 1u);
+            if (r.overflow) atomicOr(&a.ctr->q[7][0], 1u);
             if (ANY) {
                 a.occ_out[pidx] = got ? 1 : 0;
                 if (a.hit_out) a.hit_out[pidx] = make_float4(r.best, __int_as_float(got ? r.tri : -1), r.u, r.v);
@@ -482,7 +482,7 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
 template <bool ANY>
 __device__ __forceinline__ void store_hit(const TraceArgs& a, int pidx, const RayState& r, bool got)
 {
-    if (r.overflow) atomicOr(&a.ctr->q[7], 1u);
+    if (r.overflow) atomicOr(&a.ctr->q[7][0], 1u);
     if (ANY) {
         a.occ_out[pidx] = got ? 1 : 0;
         if (a.hit_out) a.hit_out[pidx] = make_float4(r.best, __int_as_float(got ? r.tri : -1), r.u, r.v);

@@ -98,20 +98,22 @@ struct BdptSoA {
 // Counters and stats in device memory.
 constexpr int kMaxXcd = 16;     // work counters per trace launch (one per queue slice)
 
-// Device-scope atomics on one cache line serialise; the work counters of
-// the queue slices are 256 bytes apart so that slices do not wait for each
-// other.  They come last, after the part the host reads back.
+// Device-scope atomics on one cache line serialise, so every counter that
+// many waves update sits on a 256-byte line of its own: the queue counts
+// (appended to by each block of a shade pass) and the work counters of the
+// trace's queue slices.  The work counters come last, after the part the
+// host reads back.
 constexpr int kWorkStride = 64;
 
 struct DevCounters {
-    uint32_t q[12];             // queue counts, work counters (see runtime)
     unsigned long long rays_ext;
     unsigned long long rays_shadow;
     unsigned long long nodes_ext;     // counting mode only: interior nodes visited
     unsigned long long tris_ext;      // counting mode only: triangle tests
     unsigned long long nodes_shadow;
     unsigned long long tris_shadow;
-    unsigned long long pad[2];
+    unsigned long long pad[26];
+    uint32_t q[12][kWorkStride];      // queue counts and flags (see runtime), q[k][0] used
     uint32_t work[2][kMaxXcd][kWorkStride];   // chunk counters of the extend / shadow traces, per queue slice
 };
 constexpr size_t kCtrHostBytes = offsetof(DevCounters, work);   // what the host reads back

@@ -51,7 +51,11 @@ struct LayoutOpts {
     int width = 4;
     float hoist_frac = 0.01f;
     int leaf_max = 10;
+    bool rebuild = false;       // the library's own SAH tree instead of the caller's
 };
+// The library's own binned-SAH tree (3 axes, leaves of at most leaf_max
+// triangles) in the reference's node format; returns the node count.
+int build_gpu_bvh(const yn_triangle* tris, int n_tri, int leaf_max, std::vector<yn_bvh_node>& out);
 int make_layout(const yn_triangle* tris, int n_tri, int n_mat, const yn_bvh_node* nodes,
                 int n_node, const LayoutOpts& opt, GpuLayout& out, std::string& err);
 
