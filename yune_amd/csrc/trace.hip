@@ -74,9 +74,30 @@ struct RayState {
 
 // rayTriangleIntersection (udpt.cl:326-390) on a stored slot; xyz only
 // (the w terms of the reference's dots are exact zeros).
+// A load from a scene array as a global-memory load.  Through a generic
+// pointer (the scene table holds them) the compiler emits FLAT loads, which
+// also count against the LDS wait counter: every access to the LDS stack
+// would then wait for the node fetches still in flight.
+// (Through a native vector type: a cast of float4* itself still copies
+// through a generic reference.)
+typedef float yn_v4f __attribute__((ext_vector_type(4)));
+typedef int yn_v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 gload(const float4* p)
+{
+    yn_v4f v = *(const __attribute__((address_space(1))) yn_v4f*)(const void*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ int4 gload(const int4* p)
+{
+    yn_v4i v = *(const __attribute__((address_space(1))) yn_v4i*)(const void*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int s, RayState& r)
 {
-    float4 A = tri[3 * s], B = tri[3 * s + 1], C = tri[3 * s + 2];
+    float4 A = gload(tri + 3 * s), B = gload(tri + 3 * s + 1), C = gload(tri + 3 * s + 2);
     // pvec = cross(dir, v1v3)
     float px = r.dy * C.z - r.dz * C.y;
     float py = r.dz * C.x - r.dx * C.z;
@@ -203,8 +224,8 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
             if (COUNT) c.nodes++;
             if (W == 2) {
                 const float4* n = nodes + 4 * (size_t)node;
-                float4 X = n[0], Y = n[1], Z = n[2];
-                int4 C = *reinterpret_cast<const int4*>(n + 3);
+                float4 X = gload(n), Y = gload(n + 1), Z = gload(n + 2);
+                int4 C = gload(reinterpret_cast<const int4*>(n + 3));
                 float e0, e1;
                 bool h0 = box_hit<EXACT>(X.x, Y.x, Z.x, X.z, Y.z, Z.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e0);
                 bool h1 = box_hit<EXACT>(X.y, Y.y, Z.y, X.w, Y.w, Z.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e1);
@@ -221,8 +242,8 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
                 else if (h1) { node = (uint32_t)C.y; continue; }
             } else {
                 const float4* n = nodes + 8 * (size_t)node;
-                float4 LX = n[0], HX = n[1], LY = n[2], HY = n[3], LZ = n[4], HZ = n[5];
-                int4 C = *reinterpret_cast<const int4*>(n + 6);
+                float4 LX = gload(n), HX = gload(n + 1), LY = gload(n + 2), HY = gload(n + 3), LZ = gload(n + 4), HZ = gload(n + 5);
+                int4 C = gload(reinterpret_cast<const int4*>(n + 6));
                 float k0, k1, k2, k3;
                 bool h0 = box_hit<EXACT>(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k0);
                 bool h1 = box_hit<EXACT>(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k1);
@@ -286,8 +307,8 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, cons
         if (COUNT) c.nodes++;
         if (W == 2) {
             const float4* n = nodes + 4 * (size_t)node;
-            float4 X = n[0], Y = n[1], Z = n[2];
-            int4 C = *reinterpret_cast<const int4*>(n + 3);
+            float4 X = gload(n), Y = gload(n + 1), Z = gload(n + 2);
+            int4 C = gload(reinterpret_cast<const int4*>(n + 3));
             float e0, e1;
             bool h0 = box_hit<EXACT>(X.x, Y.x, Z.x, X.z, Y.z, Z.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e0);
             bool h1 = box_hit<EXACT>(X.y, Y.y, Z.y, X.w, Y.w, Z.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e1);
@@ -304,8 +325,8 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, cons
             else if (h1) { node = (uint32_t)C.y; return true; }
         } else {
             const float4* n = nodes + 8 * (size_t)node;
-            float4 LX = n[0], HX = n[1], LY = n[2], HY = n[3], LZ = n[4], HZ = n[5];
-            int4 C = *reinterpret_cast<const int4*>(n + 6);
+            float4 LX = gload(n), HX = gload(n + 1), LY = gload(n + 2), HY = gload(n + 3), LZ = gload(n + 4), HZ = gload(n + 5);
+            int4 C = gload(reinterpret_cast<const int4*>(n + 6));
             float k0, k1, k2, k3;
             bool h0 = box_hit<EXACT>(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k0);
             bool h1 = box_hit<EXACT>(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k1);
