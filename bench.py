@@ -41,13 +41,13 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_ext_1m_1080p_traffic.txt"
 
 WORKLOADS = {
     "icosphere": dict(
-        metric="Mrays/s at 1M tris 1080p", W=1920, H=1080, frames=16, steps=4, mode=0,
+        # long traversals: the closest-hit and any-hit traces of an iteration overlap on two streams
+        metric="Mrays/s at 1M tris 1080p", W=1920, H=1080, frames=16, steps=4, mode=0, trace_overlap=1,
         params=dict(mis=1, max_bounces=8, rr_threshold=2),
         desc="path tracing, 48 icospheres x 20480 tris + 12-tri room + 2-tri quad light ({n} tris), 1920x1080, "
              "{f} spp per step, max depth 8, RR from bounce 3, NEE+MIS"),
     "cornell": dict(
-        # short rays: eight queue slices (one work counter per XCD), as yn_autotune picks for this scene
-        metric="Mrays/s Cornell box 512x512", W=512, H=512, frames=128, steps=8, mode=0, trace_xcd=8,
+        metric="Mrays/s Cornell box 512x512", W=512, H=512, frames=128, steps=8, mode=0,
         params=dict(mis=1, max_bounces=8, rr_threshold=2),
         desc="path tracing, Cornell box ({n} tris), 512x512, {f} spp per step, max depth 8, RR from bounce 3, "
              "NEE+MIS"),
@@ -101,7 +101,7 @@ def serve_bench(args, Y, np):
     n_jobs = args.jobs or (int(args.duration * args.rate) if serve else 64)
     jobs = serving_jobs(Y, np, len(scenes), n_jobs, args.rate if serve else 0.0)
     lat, done_t = [], []
-    with Y.Context(mem_budget=64 << 30, graph=args.graph, trace_overlap=args.trace_overlap) as ctx:
+    with Y.Context(mem_budget=64 << 30, graph=args.graph, trace_overlap=max(args.trace_overlap, 0)) as ctx:
         t0 = time.perf_counter()
         sids = [ctx.scene(s) for s in scenes]
         print(f"# {len(scenes)} scenes, {sum(len(s.tris) for s in scenes)} triangles, uploaded in "
@@ -262,7 +262,8 @@ def main():
     ap.add_argument("--max-paths", type=int, default=0, help="paths in flight per batch (0: library default)")
     ap.add_argument("--frames", type=int, default=0, help="frames per step (0: the workload's default)")
     ap.add_argument("--batch-factor", type=int, default=0, help="work per batch over paths in flight (2: regeneration)")
-    ap.add_argument("--trace-overlap", type=int, default=0, help="1: closest-hit and any-hit traces on two streams")
+    ap.add_argument("--trace-overlap", type=int, default=-1,
+                    help="1: closest-hit and any-hit traces on two streams, 0: one stream, -1: the workload's choice")
     ap.add_argument("--graph", type=int, default=0, help="1: bounce loop replayed from a HIP graph")
     ap.add_argument("--trace-xcd", type=int, default=0, help="ray-queue slices per trace launch (1: no slicing)")
     ap.add_argument("--trace-refill", type=int, default=0, help="idle lanes before a trace wave takes new rays (-1: off)")
@@ -301,7 +302,8 @@ def main():
                    leaf_max=args.leaf_max, hoist_frac=args.hoist_frac,
                    trace_grid=args.trace_grid, lds_stack=args.lds_stack, shade_waves=args.shade_waves,
                    max_paths=args.max_paths, batch_factor=args.batch_factor,
-                   trace_overlap=args.trace_overlap, graph=args.graph,
+                   trace_overlap=args.trace_overlap if args.trace_overlap >= 0 else wl.get("trace_overlap", 0),
+                   graph=args.graph,
                    trace_xcd=args.trace_xcd or wl.get("trace_xcd", 0),
                    trace_refill=args.trace_refill or wl.get("trace_refill", 0), bvh_build=args.bvh_build) as ctx:
         sid = ctx.scene(scene, nodes)
