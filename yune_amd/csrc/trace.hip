@@ -295,10 +295,14 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
 }
 
 
-// One step of a ray's traversal, the body of traverse()'s loop: an interior
-// node (descending to its nearest child) or a leaf, then a pop when nothing
-// was descended into.  Returns true while the ray has nodes left; `node`,
-// the stack and the best hit carry over between steps.
+// One step of a ray's traversal: an interior node, descending to its
+// nearest child; when that child is a leaf (or the step began at one), the
+// leaf's triangles are tested in the same step and the next node is popped.
+// The nodes and triangles are visited in traverse()'s order, so results and
+// counts are the same; only the steps are fewer (a wave that runs interior
+// and leaf lanes side by side no longer spends a step on each).  Returns true
+// while the ray has nodes left; `node`, the stack and the best hit carry over
+// between steps.
 template <int W, bool ANY, int LST, int BLOCK, bool COUNT, bool EXACT>
 __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, const float4* __restrict__ tri,
                                           uint32_t& node, RayState& r, Stack<LST, BLOCK>& st, bool& got, Counts& c)
@@ -320,9 +324,10 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, cons
                 if (e1 < e0) { nearc = (uint32_t)C.y; farc = (uint32_t)C.x; fe = e0; }
                 if (!st.push(farc, fe)) { r.overflow = 1; return false; }   // cannot happen: depth bound
                 node = nearc;
-                return true;
-            } else if (h0) { node = (uint32_t)C.x; return true; }
-            else if (h1) { node = (uint32_t)C.y; return true; }
+                if (!(node & kLeafBit)) return true;
+            } else if (h0) { node = (uint32_t)C.x; if (!(node & kLeafBit)) return true; }
+            else if (h1) { node = (uint32_t)C.y; if (!(node & kLeafBit)) return true; }
+            else goto pop;
         } else {
             const float4* n = nodes + 8 * (size_t)node;
             float4 LX = gload(n), HX = gload(n + 1), LY = gload(n + 2), HY = gload(n + 3), LZ = gload(n + 4), HZ = gload(n + 5);
@@ -358,13 +363,14 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, cons
                 if (k1 != INFINITY) ok &= st.push(c1, k1);
                 if (!ok) { r.overflow = 1; return false; }   // cannot happen: depth bound
                 node = c0;
-                return true;
-            }
+                if (!(node & kLeafBit)) return true;
+            } else goto pop;
         }
-    } else {
-        got |= leaf_test<ANY, COUNT>(tri, node, r, c);
-        if (ANY && got) return false;
     }
+    // a leaf: this iteration's node, or the nearest child just reached
+    got |= leaf_test<ANY, COUNT>(tri, node, r, c);
+    if (ANY && got) return false;
+pop:
     // pop; a deferred child whose entry now lies beyond the best hit is dropped
     for (;;) {
         if (st.sp == 0) return false;
