@@ -306,17 +306,20 @@ int make_layout(const yn_triangle* tris, int n_tri, int n_mat, const yn_bvh_node
             err = "triangle " + std::to_string(i) + " has matID out of range"; return -2;
         }
 
-    // Shading arrays, by triangle id.
-    L.tri_shade.resize((size_t)n_tri * 12);
+    // Shading arrays, by triangle id: one 64-byte record per triangle (the
+    // three normals and the material id, read together at a closest hit
+    // in one memory sector), and matID alone.
+    L.tri_shade.assign((size_t)n_tri * 16, 0.0f);
     L.tri_mat.resize(n_tri);
     for (int i = 0; i < n_tri; ++i) {
         const yn_float4* vn[3] = {&tris[i].vn1, &tris[i].vn2, &tris[i].vn3};
         for (int k = 0; k < 3; ++k) {
             float x = vn[k]->x, y = vn[k]->y, z = vn[k]->z, w = vn[k]->w;
             float inv = 1.0f / sqrtf(x * x + y * y + z * z + w * w);
-            float* o = &L.tri_shade[(size_t)i * 12 + 4 * k];
+            float* o = &L.tri_shade[(size_t)i * 16 + 4 * k];
             o[0] = x * inv; o[1] = y * inv; o[2] = z * inv; o[3] = w * inv;
         }
+        L.tri_shade[(size_t)i * 16 + 12] = u2f_bits((uint32_t)tris[i].matID);
         L.tri_mat[i] = tris[i].matID;
     }
     auto add_slot = [&](int tri) {

@@ -96,7 +96,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
     const int slot = m1.z;
     const bool cap_on = cap.prim_tri != nullptr && slot == 0;
 
-    int tri = m0.w;
+    int vmat = m0.w;                    // material of the path's current vertex
     const size_t r4 = 4 * (size_t)p;
     f4 hp = ld(P.g1[r4]), hn = ld(P.g1[r4 + 1]), rd = ld(P.g1[r4 + 2]);
     float nee_prob = __int_as_float(m1.w);
@@ -146,16 +146,16 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
             step = S_FINISH_CONST;
         } else {
             f4 o = mk(J.cam.r1.w, J.cam.r2.w, J.cam.r3.w, 1.0f);
-            tri = htri;
+            vmat = tri_material(S.tri_shade, htri);
             hp = o + d * h.x;
-            hn = shade_normal(S.tri_shade, tri, h.z, h.w);
+            hn = shade_normal(S.tri_shade, htri, h.z, h.w);
             rd = d;
             tp = mk(1, 1, 1, 1);
             direct = mk(0, 0, 0, 0);
             indirect = mk(0, 0, 0, 0);
             acc_loaded = tp_dirty = dir_dirty = ind_dirty = v_dirty = true;
             bounce = 0;
-            Mat m = load_mat(S.mats, S.tri_mat[tri]);
+            Mat m = load_mat(S.mats, vmat);
             if (!m.spec) { st |= F_DIRECT; step = S_NEE_START; }
             else { step = S_BOUNCE_SAMPLE; }
         }
@@ -169,7 +169,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
         } else {   // ST_BOUNCE: extension result (udpt.cl:488-529)
             float4 h = P.hit[p];
             int htri = __float_as_int(h.y);
-            Mat m = load_mat(S.mats, S.tri_mat[tri]);
+            Mat m = load_mat(S.mats, vmat);
             f4 nd = ld(P.g2[r4 + 2]);
             float4 bnc = P.g2[r4 + 3];
             load_acc();
@@ -180,7 +180,8 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
                 f4 o = hp + nd * kEps;
                 f4 nhp = o + nd * h.x;
                 f4 nhn = shade_normal(S.tri_shade, htri, h.z, h.w);
-                Mat nm = load_mat(S.mats, S.tri_mat[htri]);
+                const int nmat = tri_material(S.tri_shade, htri);
+                Mat nm = load_mat(S.mats, nmat);
                 indirect = indirect + tp * nm.ke;
                 ind_dirty = tp_dirty = true;
                 if (m.spec) tp = tp * bnc.z;
@@ -196,7 +197,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
                 }
                 if (step != S_FINISH) {
                     v_dirty = true;
-                    tri = htri;
+                    vmat = nmat;
                     hp = nhp;
                     hn = nhn;
                     rd = nd;
@@ -216,7 +217,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
     // makes a ray jumps past the rest.  Without a loop back-edge, values of
     // one step are not kept live through the others.
     if (step == S_NEE_START) do {   // evaluateDirectLighting (udpt.cl:535-555)
-        Mat m = load_mat(S.mats, S.tri_mat[tri]);
+        Mat m = load_mat(S.mats, vmat);
         em = m.ke;
         f4 wraw;
         if (!sample_lights(L, hp, hn, seed, lpdf, wraw) || lpdf <= 0.0f) { nee_res = em; step = S_NEE_DONE; break; }
@@ -253,7 +254,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
             code = ST_DONE;     // consumed
         }
         if (cap_on && (st & F_DIRECT) && !(st & F_CAPSH)) { cap.shadow_occ[pix] = occ; st |= F_CAPSH; }
-        Mat m = load_mat(S.mats, S.tri_mat[tri]);
+        Mat m = load_mat(S.mats, vmat);
         em = m.ke;
         f4 w_o = -rd;
         bool glossy = fix ? ((st & F_NGLOSSY) != 0) : false;
@@ -286,7 +287,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
         goto steps_done;
     } while (0);
     if (step == S_NEE_MIS) do {     // udpt.cl:599-607
-        Mat m = load_mat(S.mats, S.tri_mat[tri]);
+        Mat m = load_mat(S.mats, vmat);
         em = m.ke;
         ls = ld(P.g3[r4 + 1]);
         if (occ) { nee_res = ls + em; step = S_NEE_DONE; break; }
@@ -312,7 +313,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
     } while (0);
     if (step == S_BOUNCE_SAMPLE) do {   // udpt.cl:458-487
         if (!RP.gi || bounce >= RP.max_bounces) { step = S_FINISH; break; }
-        Mat m = load_mat(S.mats, S.tri_mat[tri]);
+        Mat m = load_mat(S.mats, vmat);
         float pdf = 1.0f, bprob = 0.0f, ior = 1.0f;
         bool glossy = false;
         f4 nd;
@@ -350,7 +351,7 @@ steps_done:
     out.done = out.kind == 0;
 
     P.g0[2 * (size_t)p] = make_int4((int)seed, st, out.kind != 0 && v_dirty ? bounce : m0.z,
-                                    out.kind != 0 && v_dirty ? tri : m0.w);
+                                    out.kind != 0 && v_dirty ? vmat : m0.w);
     if (out.kind != 0) {
         if (np_dirty) P.g0[2 * (size_t)p + 1] = make_int4(m1.x, m1.y, m1.z, __float_as_int(nee_prob));
         if (v_dirty) {

@@ -18,7 +18,7 @@ constexpr float kEps = 0.0001f;             // udpt.cl:5
 struct DevScene {
     const float4* nodes;        // width 2: 4 float4 per node, width 4: 8 float4 per node
     const float4* tri_isect;    // 3 float4 per slot: v1 (w = tri id bits), e1, e2
-    const float4* tri_shade;    // 3 float4 per triangle: normalised vertex normals
+    const float4* tri_shade;    // 4 float4 per triangle: normalised vertex normals, {matID bits, 0, 0, 0}
     const int32_t* tri_mat;     // matID per triangle
     const yn_material* mats;
     const float4* hoist_box;    // 2 float4 per hoisted slot: lo, hi
@@ -69,7 +69,7 @@ struct PathSoA {
     // Unidirectional shading state, grouped by use: a shade step reads and
     // writes whole 32/64-byte records instead of one cache line per field,
     // which matters because a queue's paths are in no particular slot order.
-    int4* g0;                   // [2 per path] {seed, state, bounce, tri}, {pix, job, slot, nee_prob bits}
+    int4* g0;                   // [2 per path] {seed, state, bounce, material of the vertex}, {pix, job, slot, nee_prob bits}
     float4* g1;                 // [4 per path] hp, hn, rd, tp
     float4* g2;                 // [4 per path] direct, indirect, nd, bnc
     float4* g3;                 // [4 per path] wi (w: light pdf), ls, mis, -

@@ -35,7 +35,7 @@ struct GpuLayout {
     int n_hoist = 0;                    // slots [0, n_hoist): tested by every ray first
     std::vector<float> hoist_box;       // per hoisted slot: lo.xyz, -, hi.xyz, -
     std::vector<float> tri_isect;       // per slot: v1 (w = id bits), e1, e2 -> 12 floats
-    std::vector<float> tri_shade;       // per triangle: normalize(vn1..3) -> 12 floats
+    std::vector<float> tri_shade;       // per triangle: normalize(vn1..3), matID bits -> 16 floats
     std::vector<int32_t> tri_mat;       // per triangle: matID
     std::vector<int32_t> slot_tri;      // per slot: triangle id
 };

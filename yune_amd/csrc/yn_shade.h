@@ -284,9 +284,15 @@ __device__ __forceinline__ void create_ray(float px, float py, int W, int H, con
 // Shading normal at (u, v) of triangle `tri` (udpt.cl:377-383).
 __device__ __forceinline__ f4 shade_normal(const float4* __restrict__ tri_shade, int tri, float u, float v)
 {
-    f4 N1 = ld(tri_shade[3 * tri]), N2 = ld(tri_shade[3 * tri + 1]), N3 = ld(tri_shade[3 * tri + 2]);
+    f4 N1 = ld(tri_shade[4 * tri]), N2 = ld(tri_shade[4 * tri + 1]), N3 = ld(tri_shade[4 * tri + 2]);
     float w = 1 - u - v;
     return normalize((N1 * w + N2 * u) + N3 * v);
+}
+
+// Material id of triangle `tri`, from the same 64-byte record as its normals.
+__device__ __forceinline__ int tri_material(const float4* __restrict__ tri_shade, int tri)
+{
+    return __float_as_int(tri_shade[4 * tri + 3].x);
 }
 
 }  // namespace yune
