@@ -414,7 +414,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     // persistent shade grids: one block of 4 waves per CU for each wave per
     // SIMD the path-tracing shade is compiled for; the bidirectional shade
     // keeps 8 blocks per CU
-    int shade_grid = bdpt ? c->n_cu * 8 : c->n_cu * c->cfg.shade_waves;
+    int shade_grid = c->n_cu * (bdpt ? 3 : c->cfg.shade_waves);   // the shade blocks that fit at once
     const bool overlap = c->cfg.trace_overlap != 0;
     // overlapping traces get disjoint halves of the stack overflow area
     if (int r = ensure_overflow(c, max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * block * (overlap ? 2 : 1)))
