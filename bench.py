@@ -3,14 +3,14 @@
 Default workload (the headline): the ~1M-triangle 1920x1080 scene (48
 icospheres of 20,480 triangles in a 10x10x10 room, 2-triangle quad light),
 path tracing with NEE+MIS, max depth 8, Russian roulette from bounce 3.
-A step is one batch of the wavefront path tracer: 16 frames of 1920x1080
-at 1 sample per pixel each (33.2M paths) run from primary rays until every
-path has ended, then folded into the progressive image; 4 steps are the 64
+A step is one batch of the wavefront path tracer: 32 frames of 1920x1080
+at 1 sample per pixel each (66.4M paths) run from primary rays until every
+path has ended, then folded into the progressive image; 2 steps are the 64
 spp of the workload.
 
 Other single-GPU workloads (--workload):
   cornell  the 36-triangle Cornell box, 512x512, NEE+MIS, depth 8, RR from
-           bounce 3; 128 frames per step, 8 steps = 1024 spp
+           bounce 3; 256 frames per step, 4 steps = 1024 spp
   bdpt     bidirectional (bdpt.cl semantics) on the Cornell box, 1280x720,
            5-vertex subpaths, MIS; 8 frames per step, 4 steps = 32 spp
 
@@ -42,12 +42,12 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_ext_1m_1080p_traffic.txt"
 WORKLOADS = {
     "icosphere": dict(
         # long traversals: the closest-hit and any-hit traces of an iteration overlap on two streams
-        metric="Mrays/s at 1M tris 1080p", W=1920, H=1080, frames=16, steps=4, mode=0, trace_overlap=1,
+        metric="Mrays/s at 1M tris 1080p", W=1920, H=1080, frames=32, steps=2, mode=0, trace_overlap=1,
         params=dict(mis=1, max_bounces=8, rr_threshold=2),
         desc="path tracing, 48 icospheres x 20480 tris + 12-tri room + 2-tri quad light ({n} tris), 1920x1080, "
              "{f} spp per step, max depth 8, RR from bounce 3, NEE+MIS"),
     "cornell": dict(
-        metric="Mrays/s Cornell box 512x512", W=512, H=512, frames=128, steps=8, mode=0,
+        metric="Mrays/s Cornell box 512x512", W=512, H=512, frames=256, steps=4, mode=0,
         params=dict(mis=1, max_bounces=8, rr_threshold=2),
         desc="path tracing, Cornell box ({n} tris), 512x512, {f} spp per step, max depth 8, RR from bounce 3, "
              "NEE+MIS"),
@@ -379,8 +379,15 @@ def main():
     }
     tr = measured_traffic() if args.workload == "icosphere" else None
     if tr is not None:
-        line["roofline"]["traffic"], line["roofline"]["traffic_per_ray"] = round(tr[0]), round(tr[1], 1)
-        line["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+        # HBM bytes per ray from the profiled FETCH_SIZE + WRITE_SIZE, times this run's rays per
+        # launch; the algorithmic bytes above count L2 / MALL hits too, so `frac` can pass 1
+        R = line["roofline"]
+        per_launch = tr[1] * R["rays_per_launch"]
+        R["traffic"], R["traffic_per_ray"] = round(per_launch), round(tr[1], 1)
+        if R["avg_launch_ms"] > 0:
+            R["measured_hbm_gbs"] = round(per_launch / (R["avg_launch_ms"] / 1e3) / 1e9, 1)
+            R["measured_hbm_frac"] = round(R["measured_hbm_gbs"] / HBM_PEAK_GBS, 4)
+        R["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         try:

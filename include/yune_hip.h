@@ -44,7 +44,7 @@ typedef struct yn_ctx yn_ctx;
 
 typedef struct yn_config {
     int32_t device;           /* HIP device ordinal                                 */
-    int32_t max_paths;        /* paths in flight per batch (0: default, 32M)        */
+    int32_t max_paths;        /* paths in flight per batch (0: default, 64M)        */
     size_t  mem_budget;       /* bytes for job buffers; 0: 64 GiB                   */
     int32_t bvh_width;        /* GPU node width 2 or 4 (0: 4)                       */
     int32_t trace_block;      /* trace kernel block size 64/128/256 (0: 256)        */

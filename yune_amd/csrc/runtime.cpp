@@ -929,7 +929,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     *out = nullptr;
     yn_config k{};
     if (cfg) k = *cfg;
-    if (k.max_paths <= 0) k.max_paths = 1 << 25;
+    if (k.max_paths <= 0) k.max_paths = 1 << 26;     // ~27 GB of wavefront state
     if (k.mem_budget == 0) k.mem_budget = (size_t)64 << 30;
     if (k.bvh_width == 0) k.bvh_width = 4;
     if (k.trace_block == 0) k.trace_block = 256;
