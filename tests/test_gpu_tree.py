@@ -62,6 +62,21 @@ def test_empty_and_single_triangle():
     assert len(one) == 1 and one[0]["child_idx"] == -1 and one[0]["vert_len"] == 1
 
 
+def test_degenerate_triangles():
+    """Coincident centroids (median split) and a NaN vertex (bin 0) build a
+    tree with every triangle in exactly one leaf."""
+    s = Y.gen_cornell()
+    same = np.repeat(s.tris[:1], 40)
+    nodes = Y.build_gpu_bvh(same, 1)
+    leaves = nodes[nodes["child_idx"] == -1]
+    assert sorted(np.concatenate([l["vert_list"][:l["vert_len"]] for l in leaves])) == list(range(40))
+    bad = s.tris.copy()
+    bad["v2"][3, 0] = np.nan
+    nodes = Y.build_gpu_bvh(bad, 2)
+    leaves = nodes[nodes["child_idx"] == -1]
+    assert sorted(np.concatenate([l["vert_list"][:l["vert_len"]] for l in leaves])) == list(range(len(bad)))
+
+
 def test_deterministic():
     s = Y.gen_icosphere_room(7, 3, 2)
     a, b = Y.build_gpu_bvh(s.tris), Y.build_gpu_bvh(s.tris)

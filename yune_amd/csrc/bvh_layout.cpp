@@ -231,8 +231,8 @@ void sah_tree(const yn_triangle* tris, int n_tri, const std::vector<uint8_t>& ho
         double best = INFINITY;
         int bax = -1, bsplit = 0;
         auto bin_of = [&](int t, int ax, float scale) {
-            int k = (int)((cen[3 * (size_t)t + ax] - cb.lo[ax]) * scale);
-            return k < 0 ? 0 : (k >= NB ? NB - 1 : k);
+            const float f = (cen[3 * (size_t)t + ax] - cb.lo[ax]) * scale;   // NaN goes to bin 0
+            return f > 0.0f ? (f < (float)NB ? (int)f : NB - 1) : 0;
         };
         for (int ax = 0; ax < 3; ++ax) {
             const float ext = cb.hi[ax] - cb.lo[ax];

@@ -837,6 +837,13 @@ int build_scene(yn_ctx* c, int sid, int width)
     lo.rebuild = c->cfg.bvh_build == 1;
     int r = make_layout(S.h_tris.data(), (int32_t)S.h_tris.size(), (int32_t)S.h_mats.size(), S.h_nodes.data(),
                         (int32_t)S.h_nodes.size(), lo, L, err);
+    if (r == 0 && lo.rebuild && L.max_depth > kMaxStack) {
+        // a pathological triangle set can make the own tree deeper than the
+        // traversal stack; the caller's tree is laid out instead
+        lo.rebuild = false;
+        r = make_layout(S.h_tris.data(), (int32_t)S.h_tris.size(), (int32_t)S.h_mats.size(), S.h_nodes.data(),
+                        (int32_t)S.h_nodes.size(), lo, L, err);
+    }
     if (r) return fail(c, r, err);
     if (L.max_depth > kMaxStack) return fail(c, YN_ERR_INVALID, "BVH deeper than the traversal stack bound");
     HIPCK(c, hipStreamSynchronize(c->stream));
