@@ -49,7 +49,7 @@ typedef struct yn_config {
     int32_t bvh_width;        /* GPU node width 2 or 4 (0: 4)                       */
     int32_t trace_block;      /* trace kernel block size 64/128/256 (0: 256)        */
     int32_t trace_chunk;      /* rays per wave per queue fetch (0: 64)              */
-    int32_t trace_grid;       /* trace blocks per launch (0: 8 per CU)              */
+    int32_t trace_grid;       /* trace blocks per launch (0: the blocks that fit)  */
     float   hoist_frac;       /* triangles whose box area exceeds this fraction of the
                                  root box's are tested by every ray instead of living
                                  in the tree (0: 0.01; < 0: off)                    */

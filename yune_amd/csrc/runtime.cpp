@@ -410,11 +410,12 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         width = sr.info.width;   // all scenes of a context share the width
     }
     int block = c->cfg.trace_block;
-    int grid = c->cfg.trace_grid > 0 ? c->cfg.trace_grid : c->n_cu * (2048 / block);
+    // default: the trace blocks that fit at once (4 SIMDs per CU)
+    int grid = c->cfg.trace_grid > 0 ? c->cfg.trace_grid
+                                     : c->n_cu * std::max(1, trace_resident_waves(c->cfg.lds_stack) * 256 / block);
     // persistent shade grids: one block of 4 waves per CU for each wave per
-    // SIMD the path-tracing shade is compiled for; the bidirectional shade
-    // keeps 8 blocks per CU
-    int shade_grid = c->n_cu * (bdpt ? 3 : c->cfg.shade_waves);   // the shade blocks that fit at once
+    // SIMD the shade is compiled for (the bidirectional shade: 3)
+    int shade_grid = c->n_cu * (bdpt ? 3 : c->cfg.shade_waves);
     const bool overlap = c->cfg.trace_overlap != 0;
     // overlapping traces get disjoint halves of the stack overflow area
     if (int r = ensure_overflow(c, max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * block * (overlap ? 2 : 1)))
@@ -1574,7 +1575,8 @@ int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri,
         ta.work = &c->ctr->work[0][0][0];
         ta.n_slices = c->cfg.trace_xcd;
         ta.refill = c->cfg.trace_refill > 0 ? c->cfg.trace_refill : 0;
-        int grid = c->n_cu * (2048 / c->cfg.trace_block);
+        int grid = c->cfg.trace_grid > 0 ? c->cfg.trace_grid
+                                         : c->n_cu * std::max(1, trace_resident_waves(c->cfg.lds_stack) * 256 / c->cfg.trace_block);
         if (int rr = ensure_overflow(c, S.info.max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * c->cfg.trace_block))
             return rr;
         ta.ovf_ref = c->ovf_ref;

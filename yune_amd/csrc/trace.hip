@@ -705,6 +705,13 @@ void launch_l(const TraceArgs& a, int width, int block, int grid, bool count, in
 // LDS part of the traversal stack: 8, 16 or 32 entries per lane.
 int trace_lds_stack(int lds) { return lds == 8 || lds == 32 ? lds : 16; }
 
+// Waves per SIMD a trace launch runs at for a configured LDS stack.
+int trace_resident_waves(int lds)
+{
+    lds = trace_lds_stack(lds);
+    return lds == 8 ? trace_waves<8>() : (lds == 32 ? trace_waves<32>() : trace_waves<16>());
+}
+
 // `stack`: traversal stack entries the scene can need; the launch is refused
 // when LDS plus the overflow area cannot hold them.
 int launch_trace(const TraceArgs& a, bool any_hit, int width, int stack, int block, int grid, bool count, int lds,

@@ -202,6 +202,7 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
 int launch_trace(const TraceArgs& a, bool any_hit, int width, int stack, int block, int grid, bool count, int lds,
                  hipStream_t stream);
 int trace_lds_stack(int lds);   // traversal stack entries kept in LDS for a configured depth
+int trace_resident_waves(int lds);   // waves per SIMD of a trace launch (the default grid fills them)
 void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t stream);
 void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, int waves, hipStream_t stream);
 void launch_generate_bdpt(const GenArgs& a, const DevCapture& cap, hipStream_t stream);
