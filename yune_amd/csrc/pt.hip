@@ -369,19 +369,21 @@ steps_done:
 }  // namespace
 
 // WAVES: the occupancy the register allocation is held to, per SIMD (the
-// kernel needs ~133 VGPRs unconstrained, 3 waves; 4 fits in 128 without
-// spills; 5 would spill).
+// kernel needs ~133 VGPRs unconstrained, 3 waves; 4 fits in 128 with a few
+// spilled; 5 and 6 spill more and are slower).  Blocks of kShadeBlockPT
+// threads walk the queues together, one atomic per block and queue.
 template <int WAVES>
-__global__ __launch_bounds__(256, WAVES) void k_shade_pt(ShadeArgs a, DevCapture cap)
+__global__ __launch_bounds__(kShadeBlockPT, WAVES) void k_shade_pt(ShadeArgs a, DevCapture cap)
 {
-    __shared__ BlockReserveLds<3, 256> rsv;
+    __shared__ BlockReserveLds<3, kShadeBlockPT> rsv;
     const uint32_t n_ext = *a.ext_in.count, n_sh = *a.sh_in.count;
     const uint32_t total = n_ext + n_sh;
     if (blockIdx.x == 0 && (int)threadIdx.x < a.n_zero) a.zero[threadIdx.x * kWorkStride] = 0;
     uint32_t* const ctr[3] = {a.ext_out.count, a.sh_out.count, a.free_count};
     int par = 0;
     // the block walks the queues together: its appends are one atomic per queue
-    for (uint32_t base = blockIdx.x * 256u; base < total; base += gridDim.x * 256u, par ^= 1) {
+    constexpr uint32_t B = kShadeBlockPT;
+    for (uint32_t base = blockIdx.x * B; base < total; base += gridDim.x * B, par ^= 1) {
         uint32_t i = base + threadIdx.x;
         Out o;
         int pid = -1;
@@ -394,7 +396,7 @@ __global__ __launch_bounds__(256, WAVES) void k_shade_pt(ShadeArgs a, DevCapture
         const bool fr = a.free_list != nullptr && pid >= 0 && o.done;
         const uint32_t n[3] = {o.kind == 1 ? 1u : 0u, o.kind == 2 ? 1u : 0u, fr ? 1u : 0u};
         uint32_t at[3];
-        block_reserve<3, 256>(rsv, par, ctr, n, at);
+        block_reserve<3, kShadeBlockPT>(rsv, par, ctr, n, at);
         if (o.kind == 1) { a.ext_out.o_t[at[0]] = o.o_t; a.ext_out.d_p[at[0]] = o.d_p; }
         if (o.kind == 2) { a.sh_out.o_t[at[1]] = o.o_t; a.sh_out.d_p[at[1]] = o.d_p; }
         if (fr) a.free_list[at[2]] = (uint32_t)pid;     // hand the slot back for regeneration
@@ -544,9 +546,9 @@ void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t s)
 }
 void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, int waves, hipStream_t s)
 {
-    if (waves == 4) hipLaunchKernelGGL(k_shade_pt<4>, dim3(grid), dim3(256), 0, s, a, cap);
-    else if (waves == 3) hipLaunchKernelGGL(k_shade_pt<3>, dim3(grid), dim3(256), 0, s, a, cap);
-    else hipLaunchKernelGGL(k_shade_pt<2>, dim3(grid), dim3(256), 0, s, a, cap);
+    if (waves == 4) hipLaunchKernelGGL(k_shade_pt<4>, dim3(grid), dim3(kShadeBlockPT), 0, s, a, cap);
+    else if (waves == 3) hipLaunchKernelGGL(k_shade_pt<3>, dim3(grid), dim3(kShadeBlockPT), 0, s, a, cap);
+    else hipLaunchKernelGGL(k_shade_pt<2>, dim3(grid), dim3(kShadeBlockPT), 0, s, a, cap);
 }
 void launch_resolve(const ResolveArgs& a, hipStream_t s)
 {

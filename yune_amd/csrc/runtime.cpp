@@ -413,9 +413,9 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     // default: the trace blocks that fit at once (4 SIMDs per CU)
     int grid = c->cfg.trace_grid > 0 ? c->cfg.trace_grid
                                      : c->n_cu * std::max(1, trace_resident_waves(c->cfg.lds_stack) * 256 / block);
-    // persistent shade grids: one block of 4 waves per CU for each wave per
-    // SIMD the shade is compiled for (the bidirectional shade: 3)
-    int shade_grid = c->n_cu * (bdpt ? 3 : c->cfg.shade_waves);
+    // persistent shade grids: the blocks that fit at once (4 SIMDs per CU;
+    // the bidirectional shade runs 3 waves per SIMD in blocks of 256)
+    int shade_grid = bdpt ? c->n_cu * 3 : std::max(1, c->n_cu * c->cfg.shade_waves * 256 / kShadeBlockPT);
     const bool overlap = c->cfg.trace_overlap != 0;
     // overlapping traces get disjoint halves of the stack overflow area
     if (int r = ensure_overflow(c, max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * block * (overlap ? 2 : 1)))
