@@ -72,8 +72,6 @@ struct RayState {
     int overflow;           // traversal stack would have overflowed
 };
 
-// rayTriangleIntersection (udpt.cl:326-390) on a stored slot; xyz only
-// (the w terms of the reference's dots are exact zeros).
 // A load from a scene array as a global-memory load.  Through a generic
 // pointer (the scene table holds them) the compiler emits FLAT loads, which
 // also count against the LDS wait counter: every access to the LDS stack
@@ -95,6 +93,8 @@ __device__ __forceinline__ int4 gload(const int4* p)
     return make_int4(v.x, v.y, v.z, v.w);
 }
 
+// rayTriangleIntersection (udpt.cl:326-390) on a stored slot; xyz only
+// (the w terms of the reference's dots are exact zeros).
 __device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int s, RayState& r)
 {
     float4 A = gload(tri + 3 * s), B = gload(tri + 3 * s + 1), C = gload(tri + 3 * s + 2);
