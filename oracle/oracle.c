@@ -20,6 +20,14 @@
 #include <string.h>
 #include <pthread.h>
 
+/* Planted mistakes, for tests only: built with -DOR_PLANT=k the oracle
+ * carries mistake k (a dropped term, a wrong weight, a swapped operand; the
+ * list is in tests/test_oracle_planted.py), so a test can show that the
+ * checks which pin the oracle fail on it.  The default build, 0, has none. */
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
+
 #define PI      3.14159265359f      /* udpt.cl:3 */
 #define INV_PI  0.31830988618f      /* udpt.cl:4 */
 #define EPSILON 0.0001f             /* udpt.cl:5 */
@@ -384,6 +392,8 @@ typedef struct {
     int heap;
     uint64_t rays;      /* BVH queries issued */
     int last_trunc;
+    uint64_t truncs;    /* queries whose BFS list reached HEAP_SIZE */
+    uint64_t conn_traced, conn_occluded;   /* BDPT connection rays */
 } tctx;
 
 /* rayAabbIntersection (udpt.cl:392-431); min/max are fminf/fmaxf. */
@@ -489,6 +499,7 @@ static int traverse(tctx* c, ray_t* r, hit_t* h)
         }
     }
     c->last_trunc = len >= heap;
+    c->truncs += (uint64_t)c->last_trunc;
     return intersect;
 }
 
@@ -510,6 +521,7 @@ static void tctx_init(tctx* c, const or_scene* s)
     c->cand = (int*)malloc(sizeof(int) * (c->heap + 2));
     c->rays = 0;
     c->last_trunc = 0;
+    c->truncs = c->conn_traced = c->conn_occluded = 0;
 }
 static void tctx_free(tctx* c) { free(c->cand); }
 
@@ -622,7 +634,9 @@ static v4 eval_brdf(const yn_material* m, v4 w_i, v4 w_o, v4 p, v4 n, int glossy
     }
     float cos_alpha = ynf_powf(fmaxf(dot(w_o, refl), 0.0f), m->px + m->py);
     int phong_exp = (int)(m->px + m->py);
-    return divs(muls(muls(muls(muls(F4(m->ks), cos_alpha), (float)(phong_exp + 2)), INV_PI), 0.5f), rr_prob);
+    return divs(muls(muls(muls(muls(F4(m->ks), cos_alpha), (float)(phong_exp + (OR_PLANT == 11 ? 1 : 2))), INV_PI),
+                     0.5f),
+                rr_prob);
 }
 
 /* sampleGlossyPdf (udpt.cl:1062-1119; bdpt.cl:1048-1105 when `bdpt`) */
@@ -764,6 +778,7 @@ static inline float cos_pdf(v4 w_i, v4 n) { return fmaxf(dot(w_i, n), 0.0f) * IN
 
 static inline float power_heuristic(float w, float p1, float p2)   /* :1149-1151, beta 2 */
 {
+    if (OR_PLANT == 10) return w / (p1 + p2);
     return (w * w) / (p1 * p1 + p2 * p2);
 }
 
@@ -782,7 +797,7 @@ static int sample_lights(const yn_light_quad* L, v4 p, v4 n, uint32_t* seed, flo
     float area = length(el) * length(ew);
     *w_i = wis;
     *light_pdf = 1 / area;
-    *light_pdf *= distance / fmaxf(dot(neg(temp_wi), ln), 0.0f);
+    *light_pdf *= (OR_PLANT == 9 ? 1.0f : distance) / fmaxf(dot(neg(temp_wi), ln), 0.0f);
     return 0;
 }
 
@@ -948,6 +963,12 @@ static v4 shade_pt(tctx* c, const yn_render_params* P, ray_t ray, uint32_t* seed
 /* ------------------------------------------------------------------ */
 typedef struct { hit_t hit; v4 dir, contrib; float fwd_pdf, rev_pdf; } pathinfo;
 
+/* createLightPath (bdpt.cl:432-511).  Vertex 0 is a uniform point on the
+ * quad: r1 by an FP64 division (:439), r2 on edge_l and r1 on edge_w (:443-
+ * 444, swapped against sampleLights); contrib ke / (1/A) = ke * A.  Vertex 1
+ * comes from a cosine-weighted ray: contrib max(0, n0 . d) / pdf * contrib0
+ * (pi * ke * A up to rounding).  Later vertices: BRDF * cos / pdf *
+ * previous contrib, with Russian roulette after RR_THRESHOLD. */
 static void create_light_path(tctx* c, const yn_render_params* P, pathinfo* lp, int* len, uint32_t* seed)
 {
     const yn_light_quad* L = &c->s->light;
@@ -958,23 +979,24 @@ static void create_light_path(tctx* c, const yn_render_params* P, pathinfo* lp, 
     float r1 = (float)((double)*seed / (double)4294967295u);   /* bdpt.cl:439, FP64 */
     *seed = or_xor_shift(*seed);
     float r2 = or_u2f(*seed);
-    v4 A = muls(F4(L->edge_l), r2);
-    v4 B = muls(F4(L->edge_w), r1);
+    v4 A = muls(F4(L->edge_l), OR_PLANT == 4 ? r1 : r2);
+    v4 B = muls(F4(L->edge_w), OR_PLANT == 4 ? r2 : r1);
     hit.p = add(add(A, B), F4(L->pos));
     hit.n = F4(L->normal);
     lp[0].hit = hit;
     lp[0].hit.light = 0;
     float area = length(F4(L->edge_l)) * length(F4(L->edge_w));
-    lp[0].fwd_pdf = 1.0f / area;
+    lp[0].fwd_pdf = OR_PLANT == 5 ? 1.0f : 1.0f / area;
     lp[0].rev_pdf = 1.0f;
     lp[0].contrib = divs(F4(L->ke), lp[0].fwd_pdf);
+    lp[0].dir = V(0, 0, 0, 0);
     cosine_sample(&lr, &pdf, hit.p, hit.n, seed);
     if (pdf <= 0.0f || !trace(c, &lr, &hit) || hit.light >= 0) return;
     lp[1].hit = hit;
     lp[1].dir = lr.d;
     lp[1].fwd_pdf = pdf;
     lp[1].rev_pdf = 1.0f;
-    lp[1].contrib = bcast(fmaxf(0.0f, dot(lp[0].hit.n, lp[1].dir)) / pdf);
+    lp[1].contrib = bcast(OR_PLANT == 6 ? 1.0f : fmaxf(0.0f, dot(lp[0].hit.n, lp[1].dir)) / pdf);
     lp[1].contrib = mul(lp[1].contrib, lp[0].contrib);
     float brdf_prob = 0.0f;
     (*len)++;
@@ -1051,6 +1073,69 @@ static void create_eye_path(tctx* c, const yn_render_params* P, pathinfo* ep, in
 
 #define OR_MAX_BDPT 64
 
+/* The eye-vertex loop of shading (bdpt.cl:574-640) over given subpaths:
+ * at eye vertex i, T_i * (ke + evaluateDirectLighting) * w_i, plus the
+ * connections to light vertices j = lp_len-1 .. 1 that face each other and
+ * are unoccluded, G * f_eye * f_light * contrib_j * T_i summed and weighted
+ * w_i * (1 - ks_i); w_0 = 1, w_{i+1} = w_i * ks_i with ks_i the largest
+ * specular channel floored at 0.1.  A connection ray runs to the light
+ * vertex's exact distance, so the surface under the light vertex can
+ * occlude it by rounding; tctx counts connection rays and occlusions. */
+static v4 bdpt_combine(tctx* c, const yn_render_params* P, const pathinfo* lp, int lp_len, const pathinfo* ep,
+                       int ep_len, uint32_t* seed, edl_cap* ec)
+{
+    const or_scene* s = c->s;
+    v4 throughput, color = V(0, 0, 0, 1), subpaths;
+    float eye_path_weight = 1.0f, ks = 0.0f;
+    for (int i = 0; i < ep_len; i++) {
+        if (eye_path_weight == 0) break;
+        subpaths = V(0, 0, 0, 1);
+        const yn_material* m = mat_of(s, ep[i].hit.tri);
+        v4 spec = F4(m->ks), emission = F4(m->ke);
+        ks = fmaxf(fmaxf(spec.x, fmaxf(spec.y, spec.z)), OR_PLANT == 3 ? 0.0f : 0.1f);
+        throughput = ep[i].contrib;
+        color = add(color, muls(mul(throughput, add(emission, edl(c, P, neg(ep[i].dir), &ep[i].hit, seed, 1,
+                                                                 i == 0 ? ec : NULL))),
+                                eye_path_weight));
+        for (int j = lp_len - 1; j > 0; j--) {
+            ray_t dr;
+            hit_t dh = hit_init();
+            v4 tlp = bcast(1.0f);
+            dr.d = normalize(sub(lp[j].hit.p, ep[i].hit.p));
+            dr.o = add(ep[i].hit.p, muls(dr.d, EPSILON));
+            float dist = length(sub(lp[j].hit.p, ep[i].hit.p));
+            dist *= dist;
+            if (OR_PLANT == 1) dist = 1.0f;
+            dr.len = length(sub(lp[j].hit.p, dr.o));
+            dr.shadow = 1;
+            if (dot(dr.d, ep[i].hit.n) <= 0 || dot(neg(dr.d), lp[j].hit.n) <= 0) continue;
+            c->conn_traced++;
+            if (!trace(c, &dr, &dh)) {
+                tlp = lp[j].contrib;
+                v4 w_i = dr.d, w_o = neg(ep[i].dir);
+                const hit_t* hh = &ep[i].hit;
+                float gf = fmaxf(dot(w_i, hh->n), 0.0f) *
+                           (OR_PLANT == 7 ? 1.0f : fmaxf(dot(neg(w_i), lp[j].hit.n), 0.0f)) / dist;
+                float prob = 0.0f;
+                int g = sample_glossy_pdf(m, seed, &prob, 1);
+                v4 e2l = eval_brdf(m, w_i, w_o, hh->p, hh->n, g, prob, 0, 1);
+                const yn_material* lm = mat_of(s, lp[j].hit.tri);
+                g = sample_glossy_pdf(lm, seed, &prob, 1);
+                v4 l2e = eval_brdf(lm, neg(lp[j].dir), neg(w_i), lp[j].hit.p, lp[j].hit.n, g, prob, 0, 1);
+                if (OR_PLANT == 8) l2e = bcast(1.0f);
+                tlp = mul(tlp, mul(muls(e2l, gf), l2e));
+                tlp = mul(tlp, throughput);
+                subpaths = add(subpaths, tlp);
+            } else {
+                c->conn_occluded++;
+            }
+        }
+        color = add(color, muls(subpaths, OR_PLANT == 2 ? eye_path_weight : (eye_path_weight * (1 - ks))));
+        eye_path_weight = eye_path_weight * ks;
+    }
+    return color;
+}
+
 /* shading (bdpt.cl:562-640) */
 static v4 shade_bdpt(tctx* c, const yn_render_params* P, ray_t ray, uint32_t* seed, pix_cap* pc)
 {
@@ -1065,50 +1150,7 @@ static v4 shade_bdpt(tctx* c, const yn_render_params* P, ray_t ray, uint32_t* se
         return V(0.1f, 0.1f, 0.1f, 1);
     } else if (ep[0].hit.tri < 0)
         return V(P->background[0], P->background[1], P->background[2], P->background[3]);
-
-    v4 throughput, color = V(0, 0, 0, 1), subpaths;
-    float eye_path_weight = 1.0f, ks = 0.0f;
-    for (int i = 0; i < ep_len; i++) {
-        if (eye_path_weight == 0) break;
-        subpaths = V(0, 0, 0, 1);
-        const yn_material* m = mat_of(s, ep[i].hit.tri);
-        v4 spec = F4(m->ks), emission = F4(m->ke);
-        ks = fmaxf(fmaxf(spec.x, fmaxf(spec.y, spec.z)), 0.1f);
-        throughput = ep[i].contrib;
-        color = add(color, muls(mul(throughput, add(emission, edl(c, P, neg(ep[i].dir), &ep[i].hit, seed, 1,
-                                                                 i == 0 ? ec : NULL))),
-                                eye_path_weight));
-        for (int j = lp_len - 1; j > 0; j--) {
-            ray_t dr;
-            hit_t dh = hit_init();
-            v4 tlp = bcast(1.0f);
-            dr.d = normalize(sub(lp[j].hit.p, ep[i].hit.p));
-            dr.o = add(ep[i].hit.p, muls(dr.d, EPSILON));
-            float dist = length(sub(lp[j].hit.p, ep[i].hit.p));
-            dist *= dist;
-            dr.len = length(sub(lp[j].hit.p, dr.o));
-            dr.shadow = 1;
-            if (dot(dr.d, ep[i].hit.n) <= 0 || dot(neg(dr.d), lp[j].hit.n) <= 0) continue;
-            if (!trace(c, &dr, &dh)) {
-                tlp = lp[j].contrib;
-                v4 w_i = dr.d, w_o = neg(ep[i].dir);
-                const hit_t* hh = &ep[i].hit;
-                float gf = fmaxf(dot(w_i, hh->n), 0.0f) * fmaxf(dot(neg(w_i), lp[j].hit.n), 0.0f) / dist;
-                float prob = 0.0f;
-                int g = sample_glossy_pdf(m, seed, &prob, 1);
-                v4 e2l = eval_brdf(m, w_i, w_o, hh->p, hh->n, g, prob, 0, 1);
-                const yn_material* lm = mat_of(s, lp[j].hit.tri);
-                g = sample_glossy_pdf(lm, seed, &prob, 1);
-                v4 l2e = eval_brdf(lm, neg(lp[j].dir), neg(w_i), lp[j].hit.p, lp[j].hit.n, g, prob, 0, 1);
-                tlp = mul(tlp, mul(muls(e2l, gf), l2e));
-                tlp = mul(tlp, throughput);
-                subpaths = add(subpaths, tlp);
-            }
-        }
-        color = add(color, muls(subpaths, (eye_path_weight * (1 - ks))));
-        eye_path_weight = eye_path_weight * ks;
-    }
-    return color;
+    return bdpt_combine(c, P, lp, lp_len, ep, ep_len, seed, ec);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1159,7 +1201,7 @@ typedef struct {
     const uint32_t* seeds; int n_frames, reset_first, x0, y0, x1, y1;
     float* accum; or_capture* cap;
     int next_row; pthread_mutex_t mu;
-    uint64_t rays, samples;
+    uint64_t rays, samples, truncs, conn_traced, conn_occluded;
 } job_t;
 
 static void* worker(void* arg)
@@ -1211,6 +1253,9 @@ static void* worker(void* arg)
     pthread_mutex_lock(&J->mu);
     J->rays += c.rays;
     J->samples += samples;
+    J->truncs += c.truncs;
+    J->conn_traced += c.conn_traced;
+    J->conn_occluded += c.conn_occluded;
     pthread_mutex_unlock(&J->mu);
     tctx_free(&c);
     return NULL;
@@ -1241,7 +1286,10 @@ int or_render(const or_scene* s, const yn_cam* cam, int W, int H, const yn_rende
         free(th);
     }
     pthread_mutex_destroy(&J.mu);
-    if (rays_out) { rays_out[0] = J.rays; rays_out[1] = J.samples; }
+    if (rays_out) {
+        rays_out[0] = J.rays; rays_out[1] = J.samples; rays_out[2] = J.truncs;
+        rays_out[3] = J.conn_traced; rays_out[4] = J.conn_occluded;
+    }
     return 0;
 }
 
@@ -1361,3 +1409,61 @@ float or_phong_pdf(const yn_material* m, const float n[4], const float w_i[4], c
 float or_fm_sinf(float x) { return ynf_sinf(x); }
 float or_fm_cosf(float x) { return ynf_cosf(x); }
 float or_fm_powf(float x, float y) { return ynf_powf(x, y); }
+
+/* ---- BDPT pieces (bdpt.cl:432-640) on explicit vertices ---- */
+static void pv_out(const pathinfo* v, or_pvertex* o)
+{
+    S4(v->hit.p, o->p); S4(v->hit.n, o->n); S4(v->dir, o->dir); S4(v->contrib, o->contrib);
+    o->tri = v->hit.tri; o->light = v->hit.light; o->pad[0] = o->pad[1] = 0;
+}
+static void pv_in(const or_pvertex* o, pathinfo* v)
+{
+    memset(v, 0, sizeof *v);
+    v->hit.p = A4(o->p); v->hit.n = A4(o->n); v->dir = A4(o->dir); v->contrib = A4(o->contrib);
+    v->hit.tri = o->tri; v->hit.light = o->light;
+    v->fwd_pdf = 1.0f; v->rev_pdf = 1.0f;
+}
+
+int or_bdpt_light_path(const or_scene* s, const yn_render_params* p, uint32_t* seed, or_pvertex* out)
+{
+    if (p->bdpt_bounces > OR_MAX_BDPT) return -1;
+    tctx c;
+    tctx_init(&c, s);
+    pathinfo lp[OR_MAX_BDPT];
+    int len = 1;
+    create_light_path(&c, p, lp, &len, seed);
+    for (int i = 0; i < len; ++i) pv_out(&lp[i], &out[i]);
+    tctx_free(&c);
+    return len;
+}
+
+int or_bdpt_eye_path(const or_scene* s, const yn_render_params* p, const float o[4], const float d[4],
+                     uint32_t* seed, or_pvertex* out)
+{
+    if (p->bdpt_bounces > OR_MAX_BDPT) return -1;
+    tctx c;
+    tctx_init(&c, s);
+    pathinfo ep[OR_MAX_BDPT];
+    int len = 1;
+    ray_t r = {A4(o), A4(d), INFINITY, 0};
+    create_eye_path(&c, p, ep, &len, r, seed, NULL);
+    if (ep[0].hit.tri < 0) len = 0;     /* the camera ray found no triangle */
+    for (int i = 0; i < len; ++i) pv_out(&ep[i], &out[i]);
+    tctx_free(&c);
+    return len;
+}
+
+void or_bdpt_combine(const or_scene* s, const yn_render_params* p, const or_pvertex* lpv, int lp_len,
+                     const or_pvertex* epv, int ep_len, uint32_t* seed, float out[4], uint64_t counts[2])
+{
+    tctx c;
+    tctx_init(&c, s);
+    pathinfo lp[OR_MAX_BDPT], ep[OR_MAX_BDPT];
+    if (lp_len > OR_MAX_BDPT) lp_len = OR_MAX_BDPT;
+    if (ep_len > OR_MAX_BDPT) ep_len = OR_MAX_BDPT;
+    for (int i = 0; i < lp_len; ++i) pv_in(&lpv[i], &lp[i]);
+    for (int i = 0; i < ep_len; ++i) pv_in(&epv[i], &ep[i]);
+    S4(bdpt_combine(&c, p, lp, lp_len, ep, ep_len, seed, NULL), out);
+    if (counts) { counts[0] = c.conn_traced; counts[1] = c.conn_occluded; }
+    tctx_free(&c);
+}

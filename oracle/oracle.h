@@ -89,7 +89,9 @@ typedef struct or_capture {
  * following RendererCore::enqueueKernels/updateRenderKernelArgs: frame f
  * uses rand = frame_seeds[f]; reset on frame 0 when `reset_first`.
  * accum: W*H*4 floats (the RGBA32F image, .w = sample count).
- * rays_out (optional): [0] BVH queries issued, [1] samples.
+ * rays_out (optional, 5 entries): [0] BVH queries issued, [1] samples,
+ * [2] queries whose BFS list reached heap_size (HEAP_SIZE truncation),
+ * [3] BDPT connection rays traced, [4] of those occluded.
  * n_threads <= 0 selects 1.  cap may be NULL. */
 int or_render(const or_scene* s, const yn_cam* cam, int W, int H,
               const yn_render_params* p, const uint32_t* frame_seeds, int n_frames,
@@ -126,6 +128,27 @@ int  or_ray_tri(const yn_triangle* tri, const float o[4], const float d[4], floa
 float or_fm_sinf(float x);
 float or_fm_cosf(float x);
 float or_fm_powf(float x, float y);
+
+/* ---- BDPT pieces (bdpt.cl:432-640), for closed-form tests ----
+ * A subpath vertex (PathInfo, bdpt.cl:117-125): hit point, shading normal,
+ * direction of the ray that found it, contribution, triangle id (-1 on the
+ * light) and light id (0 for light vertex 0, else -1). */
+typedef struct or_pvertex {
+    float p[4], n[4], dir[4], contrib[4];
+    int32_t tri, light, pad[2];
+} or_pvertex;
+/* createLightPath (bdpt.cl:432-511) from *seed; returns the subpath length
+ * (vertex 0 on the light), -1 if bdpt_bounces exceeds 64.  out: 64 slots. */
+int  or_bdpt_light_path(const or_scene* s, const yn_render_params* p, uint32_t* seed, or_pvertex* out);
+/* createEyePath (bdpt.cl:513-560) from the camera ray (o, d); returns the
+ * length, 0 when the camera ray finds no triangle. */
+int  or_bdpt_eye_path(const or_scene* s, const yn_render_params* p, const float o[4], const float d[4],
+                      uint32_t* seed, or_pvertex* out);
+/* The eye-vertex loop of shading (bdpt.cl:574-640) over given subpaths:
+ * emission + direct light and all light-vertex connections, with the eye
+ * weights.  counts (optional): connection rays traced, occluded. */
+void or_bdpt_combine(const or_scene* s, const yn_render_params* p, const or_pvertex* lp, int lp_len,
+                     const or_pvertex* ep, int ep_len, uint32_t* seed, float out[4], uint64_t counts[2]);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,7 @@ baseline), never as a product path.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 
@@ -56,6 +57,12 @@ class Scene(C.Structure):
 assert Scene.light.offset == 48 and Scene.heap_size.offset == 128 and C.sizeof(Scene) == 144
 
 
+# or_pvertex (oracle.h): a BDPT subpath vertex
+PVERTEX_DTYPE = np.dtype([("p", "<f4", 4), ("n", "<f4", 4), ("dir", "<f4", 4), ("contrib", "<f4", 4),
+                          ("tri", "<i4"), ("light", "<i4"), ("pad", "<i4", 2)])
+assert PVERTEX_DTYPE.itemsize == 80
+
+
 class Capture(C.Structure):
     _fields_ = [("prim_tri", C.c_void_p), ("prim_light", C.c_void_p), ("prim_t", C.c_void_p),
                 ("shadow_ray", C.c_void_p), ("shadow_occ", C.c_void_p), ("trunc", C.c_void_p)]
@@ -69,61 +76,88 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def load(path: str) -> C.CDLL:
+    """The oracle library at `path` with its signatures declared."""
+    L = C.CDLL(path)
+    L.or_wang_hash.restype = C.c_uint32
+    L.or_wang_hash.argtypes = [C.c_uint32]
+    L.or_xor_shift.restype = C.c_uint32
+    L.or_xor_shift.argtypes = [C.c_uint32]
+    L.or_u2f.restype = C.c_float
+    L.or_u2f.argtypes = [C.c_uint32]
+    L.or_pixel_seed.restype = C.c_uint32
+    L.or_pixel_seed.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int]
+    L.or_mt19937.argtypes = [C.c_uint32, C.c_int, C.c_void_p]
+    L.or_camera.argtypes = [C.c_void_p] * 4 + [C.c_float, C.POINTER(Cam)]
+    L.or_bvh_build.restype = C.c_int
+    L.or_bvh_build.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]
+    L.or_trace.argtypes = [C.POINTER(Scene), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                           C.c_void_p]
+    L.or_render.restype = C.c_int
+    L.or_render.argtypes = [C.POINTER(Scene), C.POINTER(Cam), C.c_int, C.c_int, C.POINTER(RenderParams),
+                            C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                            C.c_void_p, C.POINTER(Capture), C.c_void_p]
+    L.or_sample.argtypes = [C.POINTER(Scene), C.POINTER(Cam), C.c_int, C.c_int, C.POINTER(RenderParams),
+                            C.c_uint32, C.c_int, C.c_int, C.c_void_p]
+    L.or_tonemap.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.or_tonemap_hable.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.or_cosine_sample.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p,
+                                   C.POINTER(C.c_float)]
+    L.or_phong_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32),
+                                  C.c_void_p, C.POINTER(C.c_float)]
+    L.or_eval_brdf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int,
+                               C.c_void_p]
+    L.or_fresnel.restype = C.c_float
+    L.or_fresnel.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
+    L.or_refract.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_sample_lights.restype = C.c_int
+    L.or_sample_lights.argtypes = [C.POINTER(LightQuad), C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32),
+                                   C.POINTER(C.c_float), C.c_void_p]
+    L.or_ray_box.restype = C.c_int
+    L.or_ray_box.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_primary_ray.argtypes = [C.POINTER(Cam), C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_void_p,
+                                 C.c_void_p]
+    L.or_phong_pdf.restype = C.c_float
+    L.or_phong_pdf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    for fn in ("or_fm_sinf", "or_fm_cosf"):
+        getattr(L, fn).restype = C.c_float
+        getattr(L, fn).argtypes = [C.c_float]
+    L.or_fm_powf.restype = C.c_float
+    L.or_fm_powf.argtypes = [C.c_float, C.c_float]
+    L.or_ray_tri.restype = C.c_int
+    L.or_ray_tri.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
+    L.or_bdpt_light_path.restype = C.c_int
+    L.or_bdpt_light_path.argtypes = [C.POINTER(Scene), C.POINTER(RenderParams), C.POINTER(C.c_uint32),
+                                     C.c_void_p]
+    L.or_bdpt_eye_path.restype = C.c_int
+    L.or_bdpt_eye_path.argtypes = [C.POINTER(Scene), C.POINTER(RenderParams), C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.c_uint32), C.c_void_p]
+    L.or_bdpt_combine.argtypes = [C.POINTER(Scene), C.POINTER(RenderParams), C.c_void_p, C.c_int, C.c_void_p,
+                                  C.c_int, C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p]
+    return L
+
+
 def lib():
     global _o
     if _o is None:
         if not os.path.exists(ORACLE_SO):
             raise ImportError(f"{ORACLE_SO} not built (make -C oracle)")
-        L = C.CDLL(ORACLE_SO)
-        L.or_wang_hash.restype = C.c_uint32
-        L.or_wang_hash.argtypes = [C.c_uint32]
-        L.or_xor_shift.restype = C.c_uint32
-        L.or_xor_shift.argtypes = [C.c_uint32]
-        L.or_u2f.restype = C.c_float
-        L.or_u2f.argtypes = [C.c_uint32]
-        L.or_pixel_seed.restype = C.c_uint32
-        L.or_pixel_seed.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int]
-        L.or_mt19937.argtypes = [C.c_uint32, C.c_int, C.c_void_p]
-        L.or_camera.argtypes = [C.c_void_p] * 4 + [C.c_float, C.POINTER(Cam)]
-        L.or_bvh_build.restype = C.c_int
-        L.or_bvh_build.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]
-        L.or_trace.argtypes = [C.POINTER(Scene), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
-                               C.c_void_p]
-        L.or_render.restype = C.c_int
-        L.or_render.argtypes = [C.POINTER(Scene), C.POINTER(Cam), C.c_int, C.c_int, C.POINTER(RenderParams),
-                                C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                C.c_void_p, C.POINTER(Capture), C.c_void_p]
-        L.or_sample.argtypes = [C.POINTER(Scene), C.POINTER(Cam), C.c_int, C.c_int, C.POINTER(RenderParams),
-                                C.c_uint32, C.c_int, C.c_int, C.c_void_p]
-        L.or_tonemap.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
-        L.or_tonemap_hable.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
-        L.or_cosine_sample.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p,
-                                       C.POINTER(C.c_float)]
-        L.or_phong_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32),
-                                      C.c_void_p, C.POINTER(C.c_float)]
-        L.or_eval_brdf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int,
-                                   C.c_void_p]
-        L.or_fresnel.restype = C.c_float
-        L.or_fresnel.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
-        L.or_refract.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.or_sample_lights.restype = C.c_int
-        L.or_sample_lights.argtypes = [C.POINTER(LightQuad), C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32),
-                                       C.POINTER(C.c_float), C.c_void_p]
-        L.or_ray_box.restype = C.c_int
-        L.or_ray_box.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
-        L.or_primary_ray.argtypes = [C.POINTER(Cam), C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_void_p,
-                                     C.c_void_p]
-        L.or_phong_pdf.restype = C.c_float
-        L.or_phong_pdf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-        for fn in ("or_fm_sinf", "or_fm_cosf"):
-            getattr(L, fn).restype = C.c_float
-            getattr(L, fn).argtypes = [C.c_float]
-        L.or_fm_powf.restype = C.c_float
-        L.or_fm_powf.argtypes = [C.c_float, C.c_float]
-        L.or_ray_tri.restype = C.c_int
-        L.or_ray_tri.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
-        _o = L
+        _o = load(ORACLE_SO)
     return _o
+
+
+@contextlib.contextmanager
+def variant(path: str):
+    """Within the block, every oracle call goes to the library at `path`
+    (a build with a planted mistake, for tests that show the pins catch
+    it)."""
+    global _o
+    saved = lib()
+    _o = load(path)
+    try:
+        yield _o
+    finally:
+        _o = saved
 
 
 def ref_available() -> bool:
@@ -209,13 +243,46 @@ class OScene:
                    "shadow_occ": np.full(n, -1, np.int32), "trunc": np.zeros(n, np.int32)}
             capst = Capture(*[_p(cap[k]).value for k in ("prim_tri", "prim_light", "prim_t", "shadow_ray",
                                                          "shadow_occ", "trunc")])
-        rays = np.zeros(2, np.uint64)
+        rays = np.zeros(8, np.uint64)
         pc = C.byref(params) if not isinstance(params, C._Pointer) else params
         r = lib().or_render(C.byref(self.s), C.byref(cam), W, H, pc, _p(seeds), len(seeds), int(reset_first),
                             x0, y0, x1, y1, threads, _p(accum), C.byref(capst) if capst else None, _p(rays))
         if r != 0:
             raise ValueError("or_render failed")
+        self.last_counts = {"truncated": int(rays[2]), "conn_traced": int(rays[3]), "conn_occluded": int(rays[4])}
         return accum, cap, int(rays[0]), int(rays[1])
+
+    def bdpt_light_path(self, params, seed: int):
+        """createLightPath from `seed`: (vertices, seed after)."""
+        out = np.zeros(64, PVERTEX_DTYPE)
+        sd = C.c_uint32(seed)
+        n = lib().or_bdpt_light_path(C.byref(self.s), C.byref(params), C.byref(sd), _p(out))
+        if n < 0:
+            raise ValueError("bad BDPT parameters")
+        return out[:n].copy(), sd.value
+
+    def bdpt_eye_path(self, params, o, d, seed: int):
+        """createEyePath from the camera ray (o, d): (vertices, seed after)."""
+        out = np.zeros(64, PVERTEX_DTYPE)
+        sd = C.c_uint32(seed)
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        n = lib().or_bdpt_eye_path(C.byref(self.s), C.byref(params), _p(o), _p(d), C.byref(sd), _p(out))
+        if n < 0:
+            raise ValueError("bad BDPT parameters")
+        return out[:n].copy(), sd.value
+
+    def bdpt_combine(self, params, lp, ep, seed: int):
+        """The eye-vertex loop of bdpt.cl shading over given subpaths:
+        (colour, seed after, connection rays traced, occluded)."""
+        lp = np.ascontiguousarray(lp, PVERTEX_DTYPE)
+        ep = np.ascontiguousarray(ep, PVERTEX_DTYPE)
+        out = np.zeros(4, np.float32)
+        cnt = np.zeros(2, np.uint64)
+        sd = C.c_uint32(seed)
+        lib().or_bdpt_combine(C.byref(self.s), C.byref(params), _p(lp), len(lp), _p(ep), len(ep), C.byref(sd),
+                              _p(out), _p(cnt))
+        return out, sd.value, int(cnt[0]), int(cnt[1])
 
 
 def params(mode=0, **over) -> RenderParams:

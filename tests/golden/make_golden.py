@@ -11,7 +11,15 @@ oracle/_ref by `make -C oracle ref`), run on:
     geometry/cornellbox-teapot.obj, parsed by the independent reader in
     tests/objread.py.
 
+It also writes cornellbox_scene.npz: geometry/cornellbox.obj with its .mtl
+(the reference's own input fixture) as the kernel arrays Scene::loadModel
+makes of it (triangles, materials, light quad, camera; read with
+yn_load_obj, whose parsing tests/test_scene_io.py checks against the
+independent reader), so GPU tests can render the reference's scene where
+the reference checkout is absent.
+
 Run from the repository root:  python tests/golden/make_golden.py
+(`... make_golden.py scenes` writes only cornellbox_scene.npz)
 """
 import hashlib
 import json
@@ -62,7 +70,15 @@ def cases():
         yield name, objread.read_obj(path)[0], {"reference_fixture": "geometry/" + name}
 
 
+def write_scenes():
+    s = Y.load_obj(os.path.join(REF, "geometry", "cornellbox.obj"))
+    np.savez_compressed(os.path.join(HERE, "cornellbox_scene.npz"), tris=s.tris, mats=s.mats,
+                        light=np.frombuffer(bytes(s.light), np.uint8), cam=np.frombuffer(bytes(s.cam), np.uint8))
+    print("cornellbox_scene.npz", len(s.tris), "triangles", len(s.mats), "materials")
+
+
 def main():
+    write_scenes()
     if not O.ref_available():
         sys.exit("oracle/_ref/libref_bvh.so is missing: make -C oracle ref (needs the reference checkout)")
     out = {"note": "expected BVH node arrays from the reference's own BVH.cpp (oracle/_ref), bins=20",
@@ -82,4 +98,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["scenes"]:
+        write_scenes()
+    else:
+        main()

@@ -101,9 +101,10 @@ def test_bdpt_reference_defaults_and_row_bands():
     assert e <= 0.01, e
 
 
-def box_scene(a, e):
+def box_scene(a, e, light_ke=(5, 5, 5)):
     """Closed grey cube [-1,1]^3 (normals inward, albedo a, emission e);
-    the light quad sits outside, facing away, and emits nothing."""
+    the light quad emits but sits outside, facing away: neither its light
+    subpaths nor NEE from inside reach the box."""
     corners = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], float)
     faces = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
     tris = np.zeros(12, Y.TRI_DTYPE)
@@ -120,9 +121,40 @@ def box_scene(a, e):
     mats[0]["ke"] = (e, e, e, 1)
     mats[0]["ks"] = (0, 0, 0, 1)
     mats[0]["n"] = mats[0]["k"] = mats[0]["px"] = mats[0]["py"] = 1
-    light = Y.light_quad((5, 5, 5), (0, 1, 0), (0, 0, 0), (0.1, 0, 0), (0, 0, 0.1))
+    light = Y.light_quad((5, 5, 5), (0, 1, 0), light_ke, (0.1, 0, 0), (0, 0, 0.1))
     cam = Y.camera((1, 0, 0), (0, 1, 0), (0, 0, -1), (0.1, 0.2, 0.3), 60.0)
     return Y.Scene(tris, mats, light, cam)
+
+
+def closed_box_with_light():
+    """The grey cube of box_scene with the light quad inside it, under the
+    top face, facing down: light and eye subpaths never leave the box."""
+    s = box_scene(0.5, 0.0)
+    s.light = Y.light_quad((-0.2, 0.9, -0.2), (0, -1, 0), (5, 4, 3), (0.4, 0, 0), (0, 0, 0.4))
+    return s
+
+
+def test_bdpt_forty_vertex_subpaths():
+    """bdpt_bounces = 40 with roulette off in a closed box: light subpaths
+    of 40 vertices, so each eye vertex issues connection rays to light
+    vertices j >= 32 (64-bit connection masks), against the oracle."""
+    s = closed_box_with_light()
+    W, H, F = 24, 24, 2
+    kw = dict(mis=1, bdpt_bounces=40, rr_threshold=100)
+    with Y.Context(max_paths=1 << 14) as c:
+        job = c.job(c.scene(s), s.cam, W, H, Y.params_reference(Y.YN_MODE_BDPT, **kw), base_seed=19)
+        job.render(F)
+        gimg = job.read()
+        st = job.stats()
+    osc = oscene(s, s.bvh())
+    oimg, _, _, _ = osc.render(ocam(s.cam), W, H, O.params(1, **kw), O.mt19937(19, F), threads=8)
+    # 40 eye vertices per sample, each with connection rays to the ~22 of
+    # its 39 light vertices that face it
+    assert osc.last_counts["conn_traced"] > W * H * F * 40 * 15
+    assert st.rays_shadow > W * H * F * 40 * 15
+    assert identical(gimg, oimg) >= 0.999
+    e, nonfinite = mre(gimg, oimg)
+    assert e <= 0.01, e
 
 
 def test_bdpt_furnace_closed_form():

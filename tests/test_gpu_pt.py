@@ -4,8 +4,8 @@ The three parity bars (closest-hit ids and t, first-shadow-ray results, and
 per-pixel radiance) are measured on the same scenes, cameras and frame
 seeds on both sides.  The oracle runs here with an uncapped traversal list
 (heap_size = 0, the exact closest hit) except where a test says otherwise;
-the reference's HEAP_SIZE truncation cannot change anything on these
-scenes (checked in test_oracle_trace.py).
+the full-size test runs it with the reference's HEAP_SIZE = 1500 list and
+checks that no query there reached the cap.
 
 Tolerances:
   * closest hit: same triangle id, and the same t bit for bit, except for
@@ -19,6 +19,7 @@ Tolerances:
     every other float operation rounds once in the reference's order, so
     only closest-hit ties (equal t on two triangles) can make paths differ.
 """
+import os
 import time
 
 import numpy as np
@@ -28,6 +29,7 @@ import yune_amd as Y
 from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def oscene(s, nodes, heap=0):
@@ -195,9 +197,12 @@ def test_radiance_icosphere_room():
 def test_full_size_scene_matches_oracle_on_a_tile():
     """The headline configuration at its full size: 983,054 triangles,
     1920x1080, NEE+MIS, depth 8, RR from bounce 3.  Two frames on the GPU
-    (its own SAH tree, default launch options) against the oracle (the
-    reference's tree, uncapped traversal) on the 96x96 centre tile of the
-    same two frames: primary hits of the first frame and pixels."""
+    (its own SAH tree, default launch options) against the oracle walking
+    the reference's tree as the reference does (breadth-first list capped
+    at HEAP_SIZE = 1500, udpt.cl:8, 288-324) on the 96x96 centre tile of the
+    same two frames: primary hits of the first frame and pixels.  No query
+    of the tile reaches the cap, so the capped walk finds the exact closest
+    hits the GPU's traversal finds."""
     s = Y.gen_icosphere_room(2024, 48, 5)
     nodes = s.bvh()
     W, H, F = 1920, 1080, 2
@@ -209,8 +214,10 @@ def test_full_size_scene_matches_oracle_on_a_tile():
         gimg = job.read()
     x0, y0 = W // 2 - 48, H // 2 - 48
     rect = (x0, y0, x0 + 96, y0 + 96)
-    oimg, ocap, _, _ = oscene(s, nodes).render(ocam(s.cam), W, H, O.params(0, **kw), O.mt19937(5, F), rect=rect,
-                                               threads=8, capture=True)
+    osc = oscene(s, nodes, heap=1500)
+    oimg, ocap, rays, _ = osc.render(ocam(s.cam), W, H, O.params(0, **kw), O.mt19937(5, F), rect=rect, threads=8,
+                                     capture=True)
+    assert rays > 96 * 96 * F and osc.last_counts["truncated"] == 0, osc.last_counts
     tile = np.zeros((H, W), bool)
     tile[y0:y0 + 96, x0:x0 + 96] = True
     t = tile.reshape(-1)
@@ -226,6 +233,71 @@ def test_full_size_scene_matches_oracle_on_a_tile():
     # the whole frame: every pixel has its samples and a finite colour
     assert (gimg[..., 3] == F).all()
     assert np.isfinite(gimg).all(-1).mean() >= 0.999
+
+
+@pytest.mark.parametrize("sigma2", [0.0, 0.1, 0.5])
+def test_radiance_oren_nayar(sigma2):
+    """Oren-Nayar diffuse (udpt-primitives.cl:673-725, 1147-1210; used by
+    evaluateBRDF for a pure diffuse lobe, rr_prob 1) with sigma^2 =
+    alpha_x on the Cornell box's diffuse walls, against the oracle.  For
+    sigma^2 > 0 the image differs from Lambert's, so the branch ran."""
+    s = Y.gen_cornell()
+    diffuse = (s.mats["ks"][:, :3].max(1) == 0) & (s.mats["is_specular"] == 0)
+    s.mats["alpha_x"][diffuse] = sigma2
+    W, H, F = 64, 64, 16
+    kw = dict(mis=1, max_bounces=8, rr_threshold=2)
+    with Y.Context(max_paths=1 << 16) as c:
+        sid = c.scene(s)
+        job = c.job(sid, s.cam, W, H, Y.params_reference(oren_nayar=1, **kw), base_seed=13)
+        job.render(F)
+        gimg = job.read()
+        lam = c.job(sid, s.cam, W, H, Y.params_reference(**kw), base_seed=13)
+        lam.render(F)
+        limg = lam.read()
+    oimg, _, _, _ = oscene(s, s.bvh()).render(ocam(s.cam), W, H, O.params(0, oren_nayar=1, **kw), O.mt19937(13, F),
+                                              threads=8)
+    assert np.array_equal(gimg[..., 3], oimg[..., 3])
+    assert identical(gimg, oimg) >= 0.999
+    e, nonfinite = mre(gimg, oimg)
+    assert e <= 0.01 and nonfinite <= 0.001, (e, nonfinite)
+    if sigma2 > 0:
+        assert mre(gimg, limg)[0] > 0.01
+
+
+def reference_cornellbox():
+    """geometry/cornellbox.obj + .mtl of the reference (diffuse walls with
+    newMaterial's alpha_x = 100, a glossy box, a mirror and a
+    mirror-transmissive material), as the arrays Scene::loadModel makes:
+    tests/golden/cornellbox_scene.npz."""
+    d = np.load(os.path.join(GOLDEN, "cornellbox_scene.npz"))
+    return Y.Scene(d["tris"].copy(), d["mats"].copy(), Y.LightQuad.from_buffer_copy(d["light"].tobytes()),
+                   Y.Cam.from_buffer_copy(d["cam"].tobytes()))
+
+
+@pytest.mark.parametrize("mode,kw", [(Y.YN_MODE_PT, dict()), (Y.YN_MODE_PT, dict(mis=1, max_bounces=8, rr_threshold=2)),
+                                     (Y.YN_MODE_PT, dict(mis=1, max_bounces=8, oren_nayar=1)),
+                                     (Y.YN_MODE_BDPT, dict(mis=1, bdpt_bounces=5))])
+def test_reference_cornellbox_obj_matches_oracle(mode, kw):
+    """The reference's own scene file, rendered on the GPU and by the
+    oracle walking the reference's BVH of it (the golden node array)."""
+    s = reference_cornellbox()
+    nodes = np.load(os.path.join(GOLDEN, "cornellbox_nodes.npz"))["nodes"]
+    W, H, F = 80, 60, 8
+    with Y.Context(max_paths=1 << 16) as c:
+        job = c.job(c.scene(s, nodes), s.cam, W, H, Y.params_reference(mode, **kw), base_seed=17)
+        cap = job.capture_next()
+        job.render(F)
+        gimg = job.read()
+    oimg, ocap, _, _ = oscene(s, nodes, heap=1500).render(ocam(s.cam), W, H, O.params(mode, **kw),
+                                                          O.mt19937(17, F), threads=8, capture=True)
+    assert np.array_equal(cap["prim_tri"], ocap["prim_tri"])
+    assert np.array_equal(gimg[..., 3], oimg[..., 3])
+    assert identical(gimg, oimg) >= 0.999
+    e, nonfinite = mre(gimg, oimg)
+    assert e <= 0.01 and nonfinite <= 0.001, (e, nonfinite)
+    # the scene's specular and glossy materials are seen
+    mats = s.mats[s.tris["matID"][cap["prim_tri"][cap["prim_tri"] >= 0]]]
+    assert (mats["is_specular"] == 1).any() and (mats["ks"][:, 0] > 0).any()
 
 
 def test_reference_defaults_unbounded_loop():

@@ -554,14 +554,31 @@ def test_trace_matches_brute_force():
 
 
 def test_bdpt_furnace_closed_form():
-    """bdpt.cl shading in an emissive grey box whose light emits nothing:
-    every eye vertex i adds T_i (ke + evaluateDirectLighting) w_i with
-    T_i = a^i, w_i = 0.1^i (ks floor 0.1, bdpt.cl:592) and NEE returning
-    the emission, so B eye vertices give 2e (1 - (0.1 a)^B) / (1 - 0.1 a)."""
+    """bdpt.cl shading in an emissive grey box.  The light quad emits but
+    sits outside the box facing away, so no light subpath vertex reaches it
+    (lp_len = 1: no connections) and NEE from inside returns the emission
+    only.  Every eye vertex i adds T_i (ke + evaluateDirectLighting) w_i
+    with T_i = a^i, w_i = 0.1^i (ks floor 0.1, bdpt.cl:592), so B eye
+    vertices give 2e (1 - (0.1 a)^B) / (1 - 0.1 a).  Light vertices and
+    connections are pinned in tests/test_oracle_bdpt.py."""
     a, e, B = 0.5, 0.2, 5
-    s = box_scene(a, e)
+    s = box_scene(a, e, light_ke=(5, 5, 5))
     cam = O.camera((1, 0, 0), (0, 1, 0), (0, 0, -1), (0.1, 0.2, 0.3), 60.0)
     p = O.params(1, bdpt_bounces=B, rr_threshold=100)
     img, _, _, _ = s.render(cam, 16, 16, p, O.mt19937(2, 2))
     exp = 2 * e * (1 - (0.1 * a) ** B) / (1 - 0.1 * a)
     assert np.allclose(img[..., :3], exp, rtol=2e-5)
+
+
+def test_heap_size_truncation_is_counted():
+    """The reference's breadth-first list stops at HEAP_SIZE entries
+    (udpt.cl:8, 288-324); the oracle counts such queries.  A list of 3
+    entries truncates most queries of the Cornell box; 1500 none."""
+    s = Y.gen_cornell()
+    cam = O.Cam.from_buffer_copy(bytes(s.cam))
+    counts = {}
+    for heap in (3, 1500):
+        osc = O.OScene(s.tris, s.mats, s.bvh(), s.light, heap_size=heap)
+        _, _, rays, _ = osc.render(cam, 16, 16, O.params(0, mis=1, max_bounces=8), O.mt19937(1, 1))
+        counts[heap] = (osc.last_counts["truncated"], rays)
+    assert counts[3][0] > 0.5 * counts[3][1] and counts[1500][0] == 0, counts

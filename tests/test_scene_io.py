@@ -67,3 +67,19 @@ def test_obj_defaults_are_the_reference_kernel_constants():
     assert tuple(lq.pos) == tuple(np.float32([-0.1979, 0.92, -3.1972, 1]))     # udpt.cl:96-104
     assert tuple(lq.ke)[:3] == (16, 16, 16) and tuple(lq.edge_l)[:3] == (np.float32(0.4), 0, 0)
     assert bytes(s.cam) == bytes(Y.camera((1, 0, 0), (0, 1, 0), (0, 0, -1), (0, 0, 0), 60.0))
+
+
+def test_cornellbox_fixture_is_the_reference_scene():
+    """tests/golden/cornellbox_scene.npz (what GPU tests render where the
+    reference checkout is absent) holds geometry/cornellbox.obj as loaded:
+    the same triangle bytes the golden BVH case was built from."""
+    import hashlib
+    import json
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    d = np.load(os.path.join(here, "cornellbox_scene.npz"))
+    case = json.load(open(os.path.join(here, "bvh_golden.json")))["cases"]["cornellbox.obj"]
+    assert hashlib.sha256(np.ascontiguousarray(d["tris"]).tobytes()).hexdigest() == case["tris_sha256"]
+    if os.path.isdir(REF):
+        s = Y.load_obj(os.path.join(REF, "cornellbox.obj"))
+        assert bytes(s.tris) == d["tris"].tobytes() and bytes(s.mats) == d["mats"].tobytes()
+        assert bytes(s.light) == d["light"].tobytes() and bytes(s.cam) == d["cam"].tobytes()

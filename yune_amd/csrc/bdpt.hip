@@ -96,7 +96,7 @@ __device__ __forceinline__ bool conn_ray(const Vtx& e, const Vtx& l, f4& o, f4& 
 
 struct BOut {
     bool ext = false;       // one extension ray (o_t, d_p)
-    uint32_t emit = 0;      // shadow-ray slots to emit this round
+    uint64_t emit = 0;      // shadow-ray slots to emit this round (bit j: slot j, j < 64)
     float4 o_t, d_p;        // the extension ray, or the slot-0 shadow ray
     int eye_i = 0;          // eye vertex of the round (slots >= 1 are connections)
 };
@@ -128,7 +128,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
     int lp_len = B.lp_len[p], ep_len = B.ep_len[p];
     float w_eye = B.w_eye[p];
     f4 color = ld(P.direct[p]);
-    uint32_t conn = B.conn[p];
+    uint64_t conn = B.conn[p];
 
     enum Step { T_LIGHT_NEXT, T_EYE_START, T_EYE_NEXT, T_ROUND_START, T_NEE_SHADOW, T_NEE_MIS, T_CONNECT,
                 T_FINISH, T_FINISH_CONST };
@@ -285,7 +285,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
                 P.nee_prob[p] = nprob;
                 float tq;
                 if (quad_hit(L, so, wi, len, tq)) st |= BF_NEE_Q;
-                else { conn |= 1u; set_ray(out, so, wi, len, p << B.shift); }
+                else { conn |= 1ull; set_ray(out, so, wi, len, p << B.shift); }
             }
             // connection rays (shading :597-609)
             for (int j = lp_len - 1; j > 0; --j) {
@@ -293,7 +293,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
                 f4 o, d;
                 float cl;
                 float tq;
-                if (conn_ray(e, l, o, d, cl) && !quad_hit(L, o, d, cl, tq)) conn |= 1u << j;
+                if (conn_ray(e, l, o, d, cl) && !quad_hit(L, o, d, cl, tq)) conn |= 1ull << j;
             }
             if (conn == 0) { step = T_NEE_SHADOW; break; }
             st = (st & ~B_MASK) | B_ROUND;
@@ -337,8 +337,8 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
             P.mis[p] = make_float4(bd.x, bd.y, bd.z, bpdf);
             P.nee_prob[p] = nprob;
             st = (st & ~B_MASK) | B_MIS;
-            conn |= 1u;
-            out.emit = 1u;
+            conn |= 1ull;
+            out.emit = 1ull;
             out.eye_i = k;
             set_ray(out, bo, bd, tq, p << B.shift);
             goto emitted;
@@ -373,7 +373,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
             color = color + (tp * (em + nee_res)) * w_eye;
             f4 sub = mk(0, 0, 0, 1);
             for (int j = lp_len - 1; j > 0; --j) {
-                if (!((conn >> j) & 1u)) continue;
+                if (!((conn >> j) & 1ull)) continue;
                 if (B.occ[((size_t)p << B.shift) + (size_t)j]) continue;
                 Vtx l = load_v(B, 0, j, p);
                 f4 d = normalize(l.p - e.p);
@@ -440,25 +440,25 @@ __global__ __launch_bounds__(256, 3) void k_shade_bdpt(BdptShadeArgs a, DevCaptu
             o = advance_bdpt(a, cap, p);
         }
         const bool more = o.ext || o.emit != 0;
-        const uint32_t cnt[3] = {o.ext ? 1u : 0u, (uint32_t)__popc(o.emit), more ? 1u : 0u};
+        const uint32_t cnt[3] = {o.ext ? 1u : 0u, (uint32_t)__popcll(o.emit), more ? 1u : 0u};
         uint32_t at[3];
         block_reserve<3, 256>(rsv, par, ctr, cnt, at);
         if (o.ext) { a.ext_out.o_t[at[0]] = o.o_t; a.ext_out.d_p[at[0]] = o.d_p; }
         uint32_t s = at[1];
         if (o.emit) {
-            uint32_t m = o.emit;
-            if (m & 1u) {
+            uint64_t m = o.emit;
+            if (m & 1ull) {
                 a.sh_out.o_t[s] = o.o_t;
                 a.sh_out.d_p[s] = o.d_p;
                 ++s;
-                m &= ~1u;
+                m &= ~1ull;
             }
             if (m) {
                 // connection rays, rebuilt exactly as when they were counted
                 const BdptSoA& B = a.B;
                 Vtx ev = load_v(B, 1, o.eye_i, p);
                 while (m) {
-                    int j = __ffs(m) - 1;
+                    int j = __ffsll((long long)m) - 1;
                     m &= m - 1;
                     Vtx l = load_v(B, 0, j, p);
                     f4 ro, rd;

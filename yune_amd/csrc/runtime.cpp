@@ -261,9 +261,10 @@ int ensure_bdpt(yn_ctx* c, int bounces)
     int shift = 0;
     while ((1 << shift) < K) ++shift;
     const int64_t R = (int64_t)1 << shift;
-    // vertices, shadow results and queues, path lists, subpath bookkeeping,
-    // and the per-path shading fields of PathSoA used by this mode
-    const int64_t per_path = 2 * K * 64 + R * 4 + 2 * R * 32 + 2 * 4 + 3 * 4 + 2 * 16 + 6 * 4 + 6 * 16;
+    // vertices, shadow results and queues, path lists, subpath bookkeeping
+    // (lengths, eye weight, camera ray, the 64-bit connection mask), and the
+    // per-path shading fields of PathSoA used by this mode
+    const int64_t per_path = 2 * K * 64 + R * 4 + 2 * R * 32 + 2 * 4 + 3 * 4 + 2 * 16 + 8 + 6 * 4 + 6 * 16;
     int64_t cap = std::min<int64_t>(c->cap_paths, (int64_t)(kBdptBudget / (size_t)per_path));
     cap = std::min<int64_t>(cap, ((int64_t)1 << (31 - shift)) - 1);
     if (cap < 1) return fail(c, YN_ERR_NO_MEMORY, "no room for bidirectional paths");
@@ -1494,7 +1495,13 @@ int yn_pathtracer(yn_ctx* c, float* out_img, const float* in_img, const yn_cam* 
     int x0 = tw * (block % block_x), y0 = th * (block / block_x);
     int w = std::min(tw, W - x0), h = std::min(th, H - y0);
     if (w <= 0 || h <= 0) return 0;
-    if (c->pt_job_slot < 0) c->pt_job_slot = kMaxJobs - 1;
+    if (c->pt_job_slot < 0) {
+        // a job slot no live job holds, kept for this entry point from now
+        // on (yn_job_create and yn_submit skip it)
+        for (int i = kMaxJobs - 1; i >= 0 && c->pt_job_slot < 0; --i)
+            if (!c->jobs[i].live) c->pt_job_slot = i;
+        if (c->pt_job_slot < 0) return fail(c, YN_ERR_NO_MEMORY, "too many jobs: no slot for yn_pathtracer");
+    }
     int jid = c->pt_job_slot;
     JobRec& J = c->jobs[jid];
     J = JobRec();

@@ -84,7 +84,7 @@ struct BdptSoA {
     int32_t* lp_len;            // light subpath length (vertex 0 lies on the quad)
     int32_t* ep_len;            // eye subpath length
     float* w_eye;               // eye_path_weight (bdpt.cl:568)
-    uint32_t* conn;             // shadow-ray slots issued in the current round
+    uint64_t* conn;             // shadow-ray slots issued in the current round (bit j: slot j)
     float4* vp;                 // vertex position, w = triangle id bits (-1: on the light)
     float4* vn;                 // shading normal
     float4* vd;                 // direction of the ray that found the vertex
