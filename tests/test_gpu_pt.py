@@ -274,13 +274,25 @@ def reference_cornellbox():
                    Y.Cam.from_buffer_copy(d["cam"].tobytes()))
 
 
-@pytest.mark.parametrize("mode,kw", [(Y.YN_MODE_PT, dict()), (Y.YN_MODE_PT, dict(mis=1, max_bounces=8, rr_threshold=2)),
-                                     (Y.YN_MODE_PT, dict(mis=1, max_bounces=8, oren_nayar=1)),
-                                     (Y.YN_MODE_BDPT, dict(mis=1, bdpt_bounces=5))])
-def test_reference_cornellbox_obj_matches_oracle(mode, kw):
+# geometry/cornellbox.obj uses only the grey, red and green materials of its
+# .mtl; the `boxes` variants give its two boxes (triangles 0-9: shortBox,
+# 10-19: longBox, in file order) other materials of the same .mtl so that the
+# glossy, mirror and mirror-transmissive entries are rendered too.
+BOX_MATS = {"glossy/mirror-transmissive": (6, 4), "teapot/mirror": (3, 5)}
+
+
+@pytest.mark.parametrize("mode,kw,boxes", [
+    (Y.YN_MODE_PT, dict(), None),
+    (Y.YN_MODE_PT, dict(mis=1, max_bounces=8, rr_threshold=2), None),
+    (Y.YN_MODE_PT, dict(mis=1, max_bounces=8, rr_threshold=2), "glossy/mirror-transmissive"),
+    (Y.YN_MODE_PT, dict(mis=1, max_bounces=8, oren_nayar=1), "teapot/mirror"),
+    (Y.YN_MODE_BDPT, dict(mis=1, bdpt_bounces=5), "glossy/mirror-transmissive")])
+def test_reference_cornellbox_obj_matches_oracle(mode, kw, boxes):
     """The reference's own scene file, rendered on the GPU and by the
     oracle walking the reference's BVH of it (the golden node array)."""
     s = reference_cornellbox()
+    if boxes is not None:
+        s.tris["matID"][0:10], s.tris["matID"][10:20] = BOX_MATS[boxes]
     nodes = np.load(os.path.join(GOLDEN, "cornellbox_nodes.npz"))["nodes"]
     W, H, F = 80, 60, 8
     with Y.Context(max_paths=1 << 16) as c:
@@ -295,9 +307,10 @@ def test_reference_cornellbox_obj_matches_oracle(mode, kw):
     assert identical(gimg, oimg) >= 0.999
     e, nonfinite = mre(gimg, oimg)
     assert e <= 0.01 and nonfinite <= 0.001, (e, nonfinite)
-    # the scene's specular and glossy materials are seen
-    mats = s.mats[s.tris["matID"][cap["prim_tri"][cap["prim_tri"] >= 0]]]
-    assert (mats["is_specular"] == 1).any() and (mats["ks"][:, 0] > 0).any()
+    if boxes is not None:
+        # both boxes' materials are seen by primary rays
+        seen = set(s.tris["matID"][cap["prim_tri"][cap["prim_tri"] >= 0]].tolist())
+        assert set(BOX_MATS[boxes]) <= seen, seen
 
 
 def test_reference_defaults_unbounded_loop():
