@@ -272,6 +272,23 @@ int  yn_fmath_eval(yn_ctx* ctx, int32_t op, const float* x, const float* y, int3
 int  yn_trace(yn_ctx* ctx, int32_t scene_id, const float* rays, int32_t n,
               int32_t* tri, int32_t* light, float* t);
 
+/* ---- No counterpart in the reference: yn_trace for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  rays, tri, light and t are
+ * DEVICE pointers in yn_trace's layouts; n <= YN_TRACE_NOALLOC_MAX_RAYS,
+ * larger n is refused with YN_ERR_INVALID.  The call allocates nothing:
+ * its ray queues, hit records, counters and traversal-stack overflow area
+ * live in one static device array per GPU (35,663,104 bytes, held until
+ * the process ends), whose address the context takes at creation.  Every
+ * call on a GPU shares that array and it keeps the last call's contents,
+ * so calls that overlap on one GPU (two contexts, two threads) race.  A
+ * traversal deeper than the array's stack (the LDS part plus 32 entries)
+ * fails with YN_ERR_INVALID.  Synchronous on the context's stream, like
+ * yn_trace, and bit-identical to it. ---- */
+#define YN_TRACE_NOALLOC_MAX_RAYS 262144
+int  yn_trace_noalloc(yn_ctx* ctx, int32_t scene_id, const float* rays, int32_t n,
+                      int32_t* tri, int32_t* light, float* t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -139,6 +139,7 @@ SIGNATURES = {
     "yn_job_set_counting": (I32, [P, I32, I32]),
     "yn_job_capture": (I32, [P, I32, P, P, P, P, P]),
     "yn_trace": (I32, [P, I32, P, I32, P, P, P]),
+    "yn_trace_noalloc": (I32, [P, I32, P, I32, P, P, P]),
     "yn_submit": (I32, [P, I32, C.POINTER(Cam), I32, I32, C.POINTER(RenderParams), C.c_uint32, I32, PI32]),
     "yn_serve": (I32, [P, I32, PI32]),
     "yn_job_status_get": (I32, [P, I32, C.POINTER(JobStatus)]),

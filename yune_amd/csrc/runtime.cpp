@@ -134,6 +134,7 @@ struct yn_ctx {
     int64_t submit_seq = 0;
     JobRec serve_rec;                 // statistics of scheduler rounds
     int64_t batch_paths = 0;          // paths per wavefront batch (0: the whole capacity)
+    void* na_scratch = nullptr;       // device address of yn_trace_noalloc's static array on this device
 };
 
 namespace {
@@ -990,6 +991,9 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
         g_err = "hipStreamCreate failed"; delete c; return YN_ERR_HIP;
     }
     for (int i = 0; i < kEvPool; ++i) hipEventCreate(&c->ev[i]);
+    if (trace_noalloc_init(k.device, &c->na_scratch) != 0) {
+        g_err = "hipGetSymbolAddress of the no-allocation scratch array failed"; return bail(YN_ERR_HIP);
+    }
     c->scenes.resize(kMaxScenes);
     c->jobs.resize(kMaxJobs);
     if ((r = dalloc(c, &c->d_scenes, kMaxScenes)) || (r = dalloc(c, &c->d_jobs, kMaxJobs)) || (r = dalloc(c, &c->ctr, 1))) {
@@ -1609,6 +1613,39 @@ int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri,
     }
     hipFree(d_rays);
     hipFree(d_light);
+    return 0;
+}
+
+int yn_trace_noalloc(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri, int32_t* light, float* t)
+{
+    if (check_ctx(c)) return YN_ERR_INVALID;
+    if (check_scene(c, sid)) return YN_ERR_NOT_FOUND;
+    if (n < 0 || (n > 0 && (!rays || !tri || !light || !t))) return fail(c, YN_ERR_INVALID, "bad trace arguments");
+    if (n > YN_TRACE_NOALLOC_MAX_RAYS)
+        return fail(c, YN_ERR_INVALID, "yn_trace_noalloc takes at most " + std::to_string(YN_TRACE_NOALLOC_MAX_RAYS) +
+                                           " rays per call, got " + std::to_string(n));
+    if (n == 0) return 0;
+    const SceneRec& S = c->scenes[sid];
+    hipStream_t s = c->stream;
+    TraceArgs ta{};
+    ta.scenes = c->d_scenes;
+    ta.jobs = c->d_jobs;
+    ta.path_job = c->P.job;
+    ta.single_scene = sid;
+    ta.chunk = c->cfg.trace_chunk;
+    ta.n_slices = c->cfg.trace_xcd;
+    ta.refill = c->cfg.trace_refill > 0 ? c->cfg.trace_refill : 0;
+    if (trace_noalloc_enqueue(c->na_scratch, ta, c->d_scenes + sid, S.info.width, c->cfg.lds_stack, c->cfg.trace_block,
+                              rays, n, tri, light, t, s) != 0)
+        return fail(c, YN_ERR_INVALID, "trace launch refused");
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipMemcpyAsync(&c->h_ctr->q[7][0], &trace_noalloc_counters(c->na_scratch)->q[7][0], sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, s));
+    HIPCK(c, hipStreamSynchronize(s));
+    if (c->h_ctr->q[7][0])
+        return fail(c, YN_ERR_INVALID, "traversal deeper than the no-allocation stack (" +
+                                           std::to_string(trace_lds_stack(c->cfg.lds_stack) + kTraceNoAllocOvfDepth) +
+                                           " entries)");
     return 0;
 }
 

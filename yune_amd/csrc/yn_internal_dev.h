@@ -2,6 +2,7 @@
 // .hip translation units and the host runtime.
 #pragma once
 
+#include "../../include/yune_hip.h"
 #include "yn_device.h"
 #include "yn_internal.h"
 #include "yn_math.h"
@@ -202,6 +203,17 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
     return base + prefix;
 }
 #endif
+
+// yn_trace_noalloc (trace_noalloc.hip): rays per call at most, threads of
+// its trace launches, and traversal stack entries its overflow area holds
+// beyond the LDS part.
+constexpr int kTraceNoAllocMaxRays = YN_TRACE_NOALLOC_MAX_RAYS;
+constexpr int kTraceNoAllocGridThreads = 1 << 16;
+constexpr int kTraceNoAllocOvfDepth = 32;
+int trace_noalloc_init(int device, void** ptr);
+int trace_noalloc_enqueue(void* base, const TraceArgs& proto, const DevScene* scene, int width, int lds, int block,
+                          const float* rays, int n, int32_t* tri, int32_t* light, float* t, hipStream_t s);
+DevCounters* trace_noalloc_counters(void* base);
 
 int launch_trace(const TraceArgs& a, bool any_hit, int width, int stack, int block, int grid, bool count, int lds,
                  hipStream_t stream);
