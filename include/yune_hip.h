@@ -76,6 +76,9 @@ typedef struct yn_config {
                                  library's own binned SAH build over the triangles
                                  (3 axes, leaves of leaf_max); 2: the caller's
                                  tree (the reference's), re-laid out; 0: 1         */
+    int32_t serve_mode;       /* yn_serve: 0: continuous batching (path-tracing jobs
+                                 join one persistent wavefront as paths end); 1:
+                                 rounds that each run until every path has ended */
 } yn_config;
 
 /* Parameters of udpt.cl / bdpt.cl as shipped: GI on, MIS off, no bounce
@@ -203,6 +206,12 @@ typedef struct yn_stats {
     uint64_t trace_launches_ext, trace_launches_shadow, iterations;
     double   ms_total;                    /* wall time of yn_job_render calls      */
     double   ms_trace_ext, ms_trace_shadow, ms_shade, ms_other;   /* HIP events */
+    /* continuous serving (yn_serve_stats): groups of bounce iterations run,
+     * the sum of the paths in flight at each group's start, refills, groups
+     * whose planning stopped for want of samples-buffer room or piece
+     * entries, and host milliseconds spent between groups */
+    uint64_t groups, paths_in_flight, refills, plan_stalls;
+    double   ms_host;
 } yn_stats;
 int  yn_job_stats(yn_ctx* ctx, int32_t job_id, yn_stats* out);
 /* 1: next renders also count node visits and triangle tests (slower). */

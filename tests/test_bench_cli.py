@@ -1,5 +1,5 @@
 """bench.py's contract pieces that need no GPU: one GPU only, its flags, and
-the committed traffic summary it reads for the roofline."""
+the committed counter summaries it reads for the roofline rows."""
 import os
 import subprocess
 import sys
@@ -25,10 +25,29 @@ def test_help_lists_the_workloads():
         assert w in r.stdout
 
 
-def test_traffic_summary_parses():
+def test_counter_summaries_parse():
     sys.path.insert(0, ROOT)
     import bench
-    tr = bench.measured_traffic()
-    assert tr is not None
-    per_launch, per_ray = tr
-    assert per_launch > 1e9 and 100 < per_ray < 5000
+    for name, wl in bench.WORKLOADS.items():
+        v = bench.read_summary(wl["summary"])
+        assert v is not None, wl["summary"]
+        for k in ("trace_ext_bytes_per_ray", "trace_any_bytes_per_ray", "shade_bytes_per_event"):
+            assert 4 < v[k] < 20000, (name, k, v[k])
+        for k in ("trace_ext_valu_busy", "shade_valu_busy"):
+            assert 0 < v[k] <= 100, (name, k, v[k])
+
+
+def test_roofline_rows_from_a_summary():
+    """The rows scale the summary's bytes per unit by the run's units per
+    launch, and the HBM fraction is achieved / peak."""
+    sys.path.insert(0, ROOT)
+    import bench
+    summary = {"source": "x.txt", "trace_ext_bytes_per_ray": 800.0, "trace_ext_valu_busy": 70.0,
+               "trace_ext_valu_util": 46.0}
+    row = bench.kernel_row("k", "ray", units=16e6 * 10, launches=10, ms=50.0, summary=summary,
+                           key="trace_ext_bytes_per_ray", alg_bytes=2500.0, valu_key="trace_ext")
+    assert row["units_per_launch"] == 16_000_000 and row["avg_launch_ms"] == 5.0
+    assert row["traffic"] == 800 * 16_000_000
+    assert abs(row["achieved"] - 800 * 16e6 / 5e-3 / 1e9) < 0.1
+    assert abs(row["frac"] - row["achieved"] / 8000.0) < 1e-4
+    assert row["algorithmic"]["bytes_per_unit"] == 2500.0 and row["valu_busy"] == 0.7

@@ -8,7 +8,15 @@
   arithmetic depends only on its pixel, frame seed and scene, and each
   pixel's frames are folded in frame order either way;
 * admission against the memory budget holds jobs back until images are
-  released, and they still finish with the same result.
+  released, and they still finish with the same result;
+* continuous batching: jobs submitted while others are on the GPU join the
+  running wavefront as paths end (they start before the running jobs
+  finish), frames larger than the paths in flight enter in parts, and
+  every image is still the job's image alone;
+* the batched mix of the benchmark (64 jobs on 8 icosphere scenes of
+  20k-2M triangles, the benchmark's own job list with 1/16 of its spp):
+  every served image equals the job alone, and tiles of three jobs equal
+  the oracle's.
 """
 import numpy as np
 import pytest
@@ -55,9 +63,12 @@ def _jobs(scenes):
     return out
 
 
-def test_served_jobs_equal_jobs_alone():
+@pytest.mark.parametrize("serve_mode", [0, 1])
+def test_served_jobs_equal_jobs_alone(serve_mode):
+    """serve_mode 0: continuous batching (the bidirectional job runs in a
+    round of its own); 1: scheduler rounds."""
     scenes = [Y.gen_cornell(), Y.gen_icosphere_room(3, 9, 2)]
-    with Y.Context(max_paths=1 << 15) as c:
+    with Y.Context(max_paths=1 << 15, serve_mode=serve_mode) as c:
         sids = [c.scene(s) for s in scenes]
         specs = _jobs(scenes)
         jobs = [c.submit(sids[j["sid"]], scenes[j["sid"]].cam, j["W"], j["H"], j["p"], j["frames"], j["seed"])
@@ -214,3 +225,131 @@ def test_autotune_chooses_valid_config_and_keeps_results():
     oimg, _, _, _ = O.OScene(s.tris, s.mats, s.bvh(), s.light, heap_size=0).render(
         O.Cam.from_buffer_copy(bytes(s.cam)), W, H, O.params(0, **kw), O.mt19937(11, 4), threads=8)
     assert (img.view(np.int32) == oimg.view(np.int32)).all(-1).mean() >= 0.999
+
+
+def test_arrivals_join_running_work():
+    """Jobs submitted while two larger ones are in flight enter the
+    running wavefront before those finish, and every image equals the job
+    rendered alone."""
+    s = Y.gen_cornell()
+    room = Y.gen_icosphere_room(5, 6, 2)
+    p = Y.params_reference(mis=1, max_bounces=8, rr_threshold=2)
+    with Y.Context(max_paths=1 << 14, serve_mode=0) as c:
+        sids = [c.scene(s), c.scene(room)]
+        cams = [s.cam, room.cam]
+        specs = [dict(sc=0, W=128, H=96, f=6, seed=21), dict(sc=1, W=96, H=96, f=5, seed=22)]
+        jobs = [c.submit(sids[j["sc"]], cams[j["sc"]], j["W"], j["H"], p, j["f"], j["seed"]) for j in specs]
+        c.serve(1)
+        st = [j.status() for j in jobs]
+        assert all(x["frames_done"] < x["frames_target"] for x in st), st
+        late = [dict(sc=1, W=40, H=30, f=3, seed=31), dict(sc=0, W=64, H=64, f=2, seed=32),
+                dict(sc=0, W=17, H=9, f=4, seed=33)]
+        t_late = c.clock_ms()
+        jobs += [c.submit(sids[j["sc"]], cams[j["sc"]], j["W"], j["H"], p, j["f"], j["seed"]) for j in late]
+        specs += late
+        for _ in range(2000):
+            if c.serve(1) == 0:
+                break
+        st = [j.status() for j in jobs]
+        assert all(x["done"] for x in st)
+        # jobs run in submission order: the late ones enter while the
+        # running jobs' last frames are still in flight
+        last_done = max(x["t_done_ms"] for x in st[:2])
+        for x in st[2:]:
+            assert t_late <= x["t_start_ms"] < last_done, (x, last_done)
+        imgs = [j.read() for j in jobs]
+        for j in jobs:
+            j.release()
+        for img, j in zip(imgs, specs):
+            alone = c.job(sids[j["sc"]], cams[j["sc"]], j["W"], j["H"], p, base_seed=j["seed"])
+            alone.render(j["f"])
+            assert np.array_equal(img.view(np.int32), alone.read().view(np.int32))
+            alone.release()
+
+
+def test_frames_larger_than_the_paths_in_flight():
+    """A frame of 6,400 pixels with 4,096 paths in flight enters the
+    continuous loop in parts (segments starting mid-frame)."""
+    s = Y.gen_cornell()
+    p = Y.params_reference(mis=1, max_bounces=8, rr_threshold=2)
+    with Y.Context(max_paths=1 << 12, serve_mode=0) as c:
+        sid = c.scene(s)
+        jobs = [c.submit(sid, s.cam, 80, 80, p, 3, 41), c.submit(sid, s.cam, 50, 30, p, 2, 42)]
+        for _ in range(2000):
+            if c.serve(1) == 0:
+                break
+        imgs = [j.read() for j in jobs]
+        for j in jobs:
+            j.release()
+        for img, (W, H, f, seed) in zip(imgs, [(80, 80, 3, 41), (50, 30, 2, 42)]):
+            alone = c.job(sid, s.cam, W, H, p, base_seed=seed)
+            alone.render(f)
+            assert np.array_equal(img.view(np.int32), alone.read().view(np.int32))
+            alone.release()
+
+
+def test_releasing_a_job_in_flight():
+    """A served job released while its paths are in flight: the others
+    still finish with their own images."""
+    s = Y.gen_cornell()
+    p = Y.params_reference(mis=1, max_bounces=8, rr_threshold=2)
+    with Y.Context(max_paths=1 << 13, serve_mode=0) as c:
+        sid = c.scene(s)
+        a = c.submit(sid, s.cam, 96, 96, p, 8, 51)
+        b = c.submit(sid, s.cam, 40, 40, p, 3, 52)
+        c.serve(1)
+        a.release()
+        for _ in range(2000):
+            if c.serve(1) == 0:
+                break
+        assert b.status()["done"]
+        img = b.read()
+        b.release()
+        alone = c.job(sid, s.cam, 40, 40, p, base_seed=52)
+        alone.render(3)
+        assert np.array_equal(img.view(np.int32), alone.read().view(np.int32))
+
+
+def test_batched_mix_matches_alone_and_oracle():
+    import bench
+    scenes = bench.serving_scenes(Y, np)
+    specs = bench.serving_jobs(Y, np, len(scenes), 64, 0.0)
+    for j in specs:
+        j["spp"] = max(1, j["spp"] // 16)
+    print(f"\nmix: {len(specs)} jobs, scenes of {[len(x.tris) for x in scenes]} triangles, "
+          f"{sum(j['spp'] * j['W'] * j['H'] for j in specs) / 1e6:.1f}M samples", flush=True)
+    with Y.Context(mem_budget=64 << 30, serve_mode=0) as c:
+        sids = [c.scene(x) for x in scenes]
+        jobs = [c.submit(sids[j["scene"]], j["cam"], j["W"], j["H"], j["params"], j["spp"], j["seed"]) for j in specs]
+        for _ in range(100000):
+            if c.serve(4) == 0:
+                break
+        served = []
+        for jb in jobs:
+            assert jb.status()["done"]
+            served.append(jb.read())
+            jb.release()
+        for img, j in zip(served, specs):
+            alone = c.job(sids[j["scene"]], j["cam"], j["W"], j["H"], j["params"], base_seed=j["seed"])
+            alone.render(j["spp"])
+            assert np.array_equal(img.view(np.int32), alone.read().view(np.int32)), j
+            alone.release()
+    # tiles of three jobs on the three smallest scenes against the oracle
+    small = np.argsort([len(x.tris) for x in scenes])[:3]
+    for sc in small:
+        k = next(i for i, j in enumerate(specs) if j["scene"] == sc)
+        j = specs[k]
+        x = scenes[sc]
+        T = 24
+        x0, y0 = j["W"] // 2 - T // 2, j["H"] // 2 - T // 2
+        osc = O.OScene(x.tris, x.mats, x.bvh(), x.light, heap_size=0)
+        oimg, _, _, _ = osc.render(O.Cam.from_buffer_copy(bytes(j["cam"])), j["W"], j["H"], O.params(0, mis=1, max_bounces=8,
+                                   rr_threshold=2), O.mt19937(j["seed"], j["spp"]), rect=(x0, y0, x0 + T, y0 + T),
+                                   threads=8)
+        g = served[k][y0:y0 + T, x0:x0 + T]
+        o = oimg[y0:y0 + T, x0:x0 + T]
+        same = (g.view(np.int32) == o.view(np.int32)).all(-1).mean()
+        a, b = g[..., :3].astype(np.float64), o[..., :3].astype(np.float64)
+        err = float((np.abs(a - b).sum(-1) / np.maximum(np.abs(b).sum(-1), 1e-2)).mean())
+        print(f"  job {k} (scene {sc}, {len(x.tris)} triangles): tile bit-identical {same:.4f}, mean rel err {err:.2e}")
+        assert same >= 0.99 and err <= 0.01, (k, same, err)

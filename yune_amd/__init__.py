@@ -173,10 +173,10 @@ class Context:
                  trace_chunk: int = 0, trace_grid: int = 0, mem_budget: int = 0, hoist_frac: float = 0.0,
                  leaf_max: int = 0, lds_stack: int = 0, shade_waves: int = 0, batch_factor: int = 0,
                  trace_overlap: int = 0, graph: int = 0, trace_xcd: int = 0, trace_refill: int = 0,
-                 bvh_build: int = 0):
+                 bvh_build: int = 0, serve_mode: int = 0):
         cfg = Config(device, max_paths, mem_budget, bvh_width, trace_block, trace_chunk, trace_grid, hoist_frac,
                      leaf_max, lds_stack, shade_waves, batch_factor, trace_overlap, graph, trace_xcd, trace_refill,
-                     bvh_build)
+                     bvh_build, serve_mode)
         self._p = C.c_void_p()
         check(lib().yn_create(C.byref(cfg), C.byref(self._p)))
         self.cfg = cfg
@@ -265,7 +265,9 @@ class Context:
         return Job(self, jid.value, width, height)
 
     def serve(self, max_rounds: int = 1) -> int:
-        """Run up to max_rounds scheduler rounds; returns the jobs still pending."""
+        """Run up to max_rounds scheduler steps (serve_mode 0: groups of four
+        bounce iterations of the continuous loop; 1: rounds); returns the
+        jobs still pending."""
         pend = C.c_int32()
         self._ck(lib().yn_serve(self._p, max_rounds, C.byref(pend)))
         return pend.value

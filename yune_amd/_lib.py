@@ -51,7 +51,7 @@ class Config(C.Structure):
                 ("trace_grid", C.c_int32), ("hoist_frac", C.c_float), ("leaf_max", C.c_int32),
                 ("lds_stack", C.c_int32), ("shade_waves", C.c_int32), ("batch_factor", C.c_int32),
                 ("trace_overlap", C.c_int32), ("graph", C.c_int32), ("trace_xcd", C.c_int32),
-                ("trace_refill", C.c_int32), ("bvh_build", C.c_int32)]
+                ("trace_refill", C.c_int32), ("bvh_build", C.c_int32), ("serve_mode", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -60,7 +60,9 @@ class Stats(C.Structure):
                 ("nodes_shadow", C.c_uint64), ("tris_shadow", C.c_uint64),
                 ("trace_launches_ext", C.c_uint64), ("trace_launches_shadow", C.c_uint64),
                 ("iterations", C.c_uint64), ("ms_total", C.c_double), ("ms_trace_ext", C.c_double),
-                ("ms_trace_shadow", C.c_double), ("ms_shade", C.c_double), ("ms_other", C.c_double)]
+                ("ms_trace_shadow", C.c_double), ("ms_shade", C.c_double), ("ms_other", C.c_double),
+                ("groups", C.c_uint64), ("paths_in_flight", C.c_uint64), ("refills", C.c_uint64),
+                ("plan_stalls", C.c_uint64), ("ms_host", C.c_double)]
 
 
 class SceneInfo(C.Structure):

@@ -486,7 +486,7 @@ __global__ __launch_bounds__(256) void k_generate_bdpt(GenArgs a, DevCapture cap
         if (a.segs[mid].path_base <= p) lo = mid; else hi = mid - 1;
     }
     const GenSeg& g = a.segs[lo];
-    int q = p - g.path_base;
+    int q = p - g.path_base + g.first;
     int x = g.x0 + q % g.w, y = g.y0 + q / g.w;
     const DevJob& J = a.jobs[g.job];
     const DevScene& S = a.scenes[J.scene];

@@ -53,6 +53,7 @@ constexpr float kPinkR = 0.988f, kPinkG = 0.0588f, kPinkB = 0.7529f;   // udpt.c
 struct Out {
     int kind = 0;           // 0 none, 1 extension, 2 shadow
     bool done = false;      // the path ended and wrote its sample
+    int job = -1;           // the path's job-table entry
     float4 o_t;
     float4 d_p;
 };
@@ -349,6 +350,7 @@ steps_done:
         st = ST_DONE;
     }
     out.done = out.kind == 0;
+    out.job = jid;
 
     P.g0[2 * (size_t)p] = make_int4((int)seed, st, out.kind != 0 && v_dirty ? bounce : m0.z,
                                     out.kind != 0 && v_dirty ? vmat : m0.w);
@@ -376,9 +378,27 @@ template <int WAVES>
 __global__ __launch_bounds__(kShadeBlockPT, WAVES) void k_shade_pt(ShadeArgs a, DevCapture cap)
 {
     __shared__ BlockReserveLds<3, kShadeBlockPT> rsv;
+    // continuous serving: ended paths per job-table entry, gathered per block
+    // in a small table and added to the device counters once at the end
+    // (device-scope atomics on one cache line, one per wave, serialise)
+    constexpr int kDoneSlots = 16;
+    __shared__ int32_t dkey[kDoneSlots];
+    __shared__ uint32_t dval[kDoneSlots];
     const uint32_t n_ext = *a.ext_in.count, n_sh = *a.sh_in.count;
     const uint32_t total = n_ext + n_sh;
     if (blockIdx.x == 0 && (int)threadIdx.x < a.n_zero) a.zero[threadIdx.x * kWorkStride] = 0;
+    if (a.done_ctr) {
+        if (threadIdx.x < (unsigned)kDoneSlots) { dkey[threadIdx.x] = -1; dval[threadIdx.x] = 0; }
+        __syncthreads();
+    }
+    auto done_add = [&](int e, uint32_t n) {
+        int h = e & (kDoneSlots - 1);
+        for (int k = 0; k < kDoneSlots; ++k, h = (h + 1) & (kDoneSlots - 1)) {
+            const int old = atomicCAS(&dkey[h], -1, e);
+            if (old == -1 || old == e) { atomicAdd(&dval[h], n); return; }
+        }
+        atomicAdd(a.done_ctr + (e - a.done_base), n);     // table full: straight to the counter
+    };
     uint32_t* const ctr[3] = {a.ext_out.count, a.sh_out.count, a.free_count};
     int par = 0;
     // the block walks the queues together: its appends are one atomic per queue
@@ -400,6 +420,24 @@ __global__ __launch_bounds__(kShadeBlockPT, WAVES) void k_shade_pt(ShadeArgs a, 
         if (o.kind == 1) { a.ext_out.o_t[at[0]] = o.o_t; a.ext_out.d_p[at[0]] = o.d_p; }
         if (o.kind == 2) { a.sh_out.o_t[at[1]] = o.o_t; a.sh_out.d_p[at[1]] = o.d_p; }
         if (fr) a.free_list[at[2]] = (uint32_t)pid;     // hand the slot back for regeneration
+        if (a.done_ctr) {
+            // ended paths per job-table entry, one atomic per entry present
+            // in the wave (mostly one: a queue's paths come in runs of one job)
+            const bool dn = pid >= 0 && o.done;
+            unsigned long long m = __ballot(dn);
+            while (m) {
+                const int leader = __ffsll((long long)m) - 1;
+                const int e = __shfl(o.job, leader);
+                const unsigned long long g = __ballot(dn && o.job == e);
+                if ((int)__lane_id() == leader && e >= a.done_base) done_add(e, (uint32_t)__popcll(g));
+                m &= ~g;
+            }
+        }
+    }
+    if (a.done_ctr) {       // the loop is block-uniform: every thread gets here
+        __syncthreads();
+        if (threadIdx.x < (unsigned)kDoneSlots && dkey[threadIdx.x] >= 0 && dval[threadIdx.x])
+            atomicAdd(a.done_ctr + (dkey[threadIdx.x] - a.done_base), dval[threadIdx.x]);
     }
 }
 
@@ -411,11 +449,13 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a, DevCapture cap)
     const bool live = i < a.n_paths;
     const int item = a.item0 + i;
     int p = i;
-    if (a.free_list && live) p = (int)a.free_list[a.free_top - 1 - (uint32_t)i];
+    if (a.free_list && live) p = (int)a.free_list[a.free_top - (uint32_t)a.n_paths + (uint32_t)i];
     // regeneration appends to the queue behind the rays already in it
     uint32_t qpos = (uint32_t)i;
     if (a.free_list) {
-        qpos = wave_append(a.ext_out.count, live);
+        // behind the rays already queued: k_queue_reserve took the count
+        // before this launch and added n_paths to it
+        qpos = *a.queue_base + (uint32_t)i;
         if (i == 0) *a.free_count = a.free_top - (uint32_t)a.n_paths;
     }
     if (!live) return;
@@ -425,7 +465,7 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a, DevCapture cap)
         if (a.segs[mid].path_base <= item) lo = mid; else hi = mid - 1;
     }
     const GenSeg& g = a.segs[lo];
-    int q = item - g.path_base;
+    int q = item - g.path_base + g.first;
     int x = g.x0 + q % g.w, y = g.y0 + q / g.w;
     const DevJob& J = a.jobs[g.job];
     const DevScene& S = a.scenes[J.scene];
@@ -534,15 +574,43 @@ __global__ __launch_bounds__(256) void k_fmath(int op, const float* x, const flo
     out[i] = op == 0 ? ynf_sinf(x[i]) : op == 1 ? ynf_cosf(x[i]) : ynf_powf(x[i], y[i]);
 }
 
+// Free list holding every slot 0..n-1 (ascending), count n: the empty
+// wavefront of the continuous serving loop.
+__global__ __launch_bounds__(256) void k_init_free(uint32_t* list, int n, uint32_t* count)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) list[i] = (uint32_t)i;
+    if (i == 0) *count = (uint32_t)n;
+}
+
+void launch_init_free(uint32_t* list, int n, uint32_t* count, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_init_free, dim3((n + 255) / 256), dim3(256), 0, s, list, n, count);
+}
+
 void launch_fmath(int op, const float* x, const float* y, int n, float* out, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_fmath, dim3((n + 255) / 256), dim3(256), 0, s, op, x, y, n, out);
 }
 
+// Reserves n entries at the end of a queue: base = count, count += n.
+__global__ void k_queue_reserve(uint32_t* count, uint32_t n, uint32_t* base)
+{
+    if (threadIdx.x == 0) {
+        const uint32_t c = *count;
+        *base = c;
+        *count = c + n;
+    }
+}
+
 void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t s)
 {
     int grid = (a.n_paths + 255) / 256;
-    if (grid > 0) hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, a, cap);
+    if (grid <= 0) return;
+    if (a.free_list)
+        hipLaunchKernelGGL(k_queue_reserve, dim3(1), dim3(64), 0, s, a.ext_out.count, (uint32_t)a.n_paths,
+                           a.queue_base);
+    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, a, cap);
 }
 void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, int waves, hipStream_t s)
 {

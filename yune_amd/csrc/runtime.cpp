@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <deque>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -71,6 +73,7 @@ struct JobRec {
     bool admitted = true;         // accumulation image allocated
     int64_t target = 0;           // frames requested
     int64_t seq = 0;              // submission order
+    int64_t planned = 0;          // frames handed to the continuous loop (pieces)
     size_t bytes = 0;             // accumulation image bytes held
     double t_submit = 0.0, t_start = -1.0, t_done = -1.0;   // ms since context creation
 };
@@ -79,6 +82,42 @@ constexpr int kMaxScenes = 64;
 constexpr int kMaxStack = 256;      // traversal stack entries a scene may need
 constexpr int kMaxJobs = 4096;
 constexpr int kEvPool = 4096;
+// Continuous serving: a piece is a run of consecutive frames of one job with
+// its own job-table entry (kMaxJobs + piece index) and samples range.
+constexpr int kMaxPieces = 4096;
+
+struct Piece {
+    bool live = false;
+    bool done = false;            // every path of every item ended
+    bool orphan = false;          // its job was released: finish, do not fold
+    int32_t job = -1;
+    int32_t scene = -1;
+    int64_t f0 = 0;               // first frame of the job it renders
+    int32_t k = 0;                // frames
+    int64_t items = 0;            // k x W x H work items
+    int64_t issued = 0;           // items turned into paths so far
+    int64_t soff = 0;             // samples buffer offset (entries)
+    std::vector<uint32_t> seeds;  // frame seeds, in frame order
+};
+
+// The persistent wavefront of continuous serving.  Jobs are cut into pieces
+// as they are admitted; the pieces' work items enter the wavefront in order
+// whenever enough paths have ended to refill their slots (at the boundary
+// of a group of bounce iterations, not at the end of a batch), and a piece
+// is folded into its job's image as soon as all its paths have ended and
+// the job's earlier pieces are folded.
+struct StreamState {
+    bool active = false;
+    int it = 0;                       // bounce iterations so far (queue parity)
+    int64_t cap = 0;                  // paths in flight at most
+    std::vector<Piece> pieces;
+    std::deque<int> issue;            // pieces with items left to issue, in order
+    std::map<int64_t, int64_t> sfree; // free ranges of the samples buffer: offset -> length
+    uint32_t free_count = 0;          // free-list length at the last sync
+    bool free_sorted = true;
+    unsigned long long last[6] = {0, 0, 0, 0, 0, 0};   // device statistics at the last sync
+    std::vector<GenSeg> segs;         // the last refill's segments (kept until it ran)
+};
 
 }  // namespace
 
@@ -135,6 +174,9 @@ struct yn_ctx {
     JobRec serve_rec;                 // statistics of scheduler rounds
     int64_t batch_paths = 0;          // paths per wavefront batch (0: the whole capacity)
     void* na_scratch = nullptr;       // device address of yn_trace_noalloc's static array on this device
+    StreamState strm;                 // continuous serving
+    uint32_t* done_ctr = nullptr;     // per piece: paths ended (device)
+    uint32_t* h_done = nullptr;       // pinned mirror
 };
 
 namespace {
@@ -221,15 +263,16 @@ int alloc_wavefront(yn_ctx* c, int64_t n)
         r |= dalloc(c, &t, std::max<size_t>(tb, 16));
         if (t) { c->allocs.push_back(t); c->sort_tmp = t; c->sort_tmp_bytes = tb; }
     }
-    {   // a batch may hold more work than paths in flight (regeneration):
-        // its samples buffer is twice the path capacity
-        int rr = dalloc(c, &c->samples, (size_t)n * 2);
+    {   // a batch may hold more work than paths in flight (regeneration), and
+        // the continuous loop keeps a piece's samples until its last path
+        // ends: four samples per path of capacity
+        int rr = dalloc(c, &c->samples, (size_t)n * 4);
         if (rr == 0) c->allocs.push_back(c->samples);
         r |= rr;
     }
     if (r) return r < 0 ? r : YN_ERR_HIP;
     c->cap_paths = n;
-    c->sample_cap = 2 * n;
+    c->sample_cap = 4 * n;
     return 0;
 }
 
@@ -336,6 +379,169 @@ struct Region {
     int32_t x0, y0, w, h;
 };
 
+struct Span { hipEvent_t a, b; int kind; };   // kind 0: closest-hit trace, 1: any-hit trace, 2: shade
+
+void add_spans(const std::vector<Span>& spans, yn_stats& st)
+{
+    for (const Span& sp : spans) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, sp.a, sp.b);
+        if (sp.kind == 0) st.ms_trace_ext += ms;
+        else if (sp.kind == 1) st.ms_trace_shadow += ms;
+        else st.ms_shade += ms;
+    }
+}
+
+// Launch shapes and buffers shared by every bounce iteration of a batch.
+struct LoopCfg {
+    bool bdpt = false;
+    int single = -1;                  // >= 0: every path renders this scene
+    int width = 2, max_depth = 0;
+    int block = 256, grid = 1, shade_grid = 1;
+    bool overlap = false, count = false;
+    uint32_t* free_list = nullptr;    // regeneration: ended paths hand their slots back here
+    uint32_t* done_ctr = nullptr;     // continuous serving: ended paths per piece (entry - kMaxJobs)
+};
+
+// One bounce iteration's launch arguments.  Every argument follows from the
+// iteration's parity (queues ping-pong), so a group of 4 iterations is the
+// same sequence of launches each time and may be replayed from a graph.
+struct IterArgs { TraceArgs te, ts; ShadeArgs sa; BdptShadeArgs ba; };
+
+int loop_cfg(yn_ctx* c, bool bdpt, const std::vector<int>& scene_ids, bool count, LoopCfg& L)
+{
+    L = LoopCfg();
+    L.bdpt = bdpt;
+    L.count = count;
+    bool same = !scene_ids.empty();
+    for (int sc : scene_ids) same &= sc == scene_ids[0];
+    if (same) L.single = scene_ids[0];
+    for (int sc : scene_ids) {
+        const SceneRec& sr = c->scenes[sc];
+        L.max_depth = std::max(L.max_depth, sr.info.max_depth);
+        L.width = sr.info.width;      // all scenes of a context share the width
+    }
+    L.block = c->cfg.trace_block;
+    // default: the trace blocks that fit at once (4 SIMDs per CU)
+    L.grid = c->cfg.trace_grid > 0 ? c->cfg.trace_grid
+                                   : c->n_cu * std::max(1, trace_resident_waves(c->cfg.lds_stack) * 256 / L.block);
+    // persistent shade grids: the blocks that fit at once (4 SIMDs per CU;
+    // the bidirectional shade runs 3 waves per SIMD in blocks of 256)
+    L.shade_grid = bdpt ? c->n_cu * 3 : std::max(1, c->n_cu * c->cfg.shade_waves * 256 / kShadeBlockPT);
+    L.overlap = c->cfg.trace_overlap != 0;
+    // overlapping traces get disjoint halves of the stack overflow area
+    if (int r = ensure_overflow(c, L.max_depth - trace_lds_stack(c->cfg.lds_stack),
+                                (int64_t)L.grid * L.block * (L.overlap ? 2 : 1)))
+        return r;
+    if (L.max_depth > trace_lds_stack(c->cfg.lds_stack) + c->ovf_depth)
+        return fail(c, YN_ERR_INVALID, "BVH too deep for the traversal stack");
+    return 0;
+}
+
+void make_iter(yn_ctx* c, const LoopCfg& L, int cur, IterArgs& A)
+{
+    std::memset(&A, 0, sizeof A);
+    const int nxt = cur ^ 1;
+    auto q = [&](int k) { return &c->ctr->q[k][0]; };
+    RayQueue* SQ = L.bdpt ? c->ShB : c->Sh;
+    TraceArgs& ta = A.te;
+    ta.scenes = c->d_scenes;
+    ta.jobs = c->d_jobs;
+    ta.path_job = c->P.job;
+    ta.single_scene = L.single;
+    ta.chunk = c->cfg.trace_chunk;
+    ta.ctr = c->ctr;
+    ta.ovf_ref = c->ovf_ref;
+    ta.ovf_t = c->ovf_t;
+    ta.ovf_depth = c->ovf_depth;
+    // extension rays
+    ta.q = c->E[cur];
+    ta.hit_out = c->P.hit;
+    ta.work = &c->ctr->work[0][0][0];
+    ta.n_slices = c->cfg.trace_xcd;
+    ta.refill = c->cfg.trace_refill > 0 ? c->cfg.trace_refill : 0;
+    ta.zero0 = c->E[nxt].count;
+    ta.zero1 = SQ[nxt].count;
+    // shadow rays
+    TraceArgs& ts = A.ts;
+    ts = ta;
+    ts.q = SQ[cur];
+    ts.hit_out = nullptr;
+    ts.occ_out = L.bdpt ? c->B.occ : c->P.occ;
+    ts.id_shift = L.bdpt ? c->B.shift : 0;
+    ts.work = &c->ctr->work[1][0][0];
+    ts.zero0 = L.bdpt ? q(8 + nxt) : nullptr;    // the next path list
+    ts.zero1 = nullptr;
+    if (L.overlap) {        // the second half of the overflow area
+        ts.ovf_ref = c->ovf_ref + (size_t)c->ovf_depth * L.grid * L.block;
+        ts.ovf_t = c->ovf_t + (size_t)c->ovf_depth * L.grid * L.block;
+    }
+    if (L.bdpt) {
+        BdptShadeArgs& ba = A.ba;
+        ba.scenes = c->d_scenes;
+        ba.jobs = c->d_jobs;
+        ba.P = c->P;
+        ba.B = c->B;
+        ba.list_in = c->lst[cur];
+        ba.list_in_count = q(8 + cur);
+        ba.list_out = c->lst[nxt];
+        ba.list_out_count = q(8 + nxt);
+        ba.ext_out = c->E[nxt];
+        ba.sh_out = c->ShB[nxt];
+        ba.zero = &c->ctr->work[0][0][0];
+        ba.n_zero = 2 * kMaxXcd;
+    } else {
+        ShadeArgs& sa = A.sa;
+        sa.scenes = c->d_scenes;
+        sa.jobs = c->d_jobs;
+        sa.P = c->P;
+        sa.ext_in = c->E[cur];
+        sa.sh_in = c->Sh[cur];
+        sa.ext_out = c->E[nxt];
+        sa.sh_out = c->Sh[nxt];
+        sa.zero = &c->ctr->work[0][0][0];
+        sa.n_zero = 2 * kMaxXcd;
+        sa.ctr = c->ctr;
+        sa.free_list = L.free_list;
+        sa.free_count = q(10);
+        sa.done_ctr = L.done_ctr;
+        sa.done_base = kMaxJobs;      // continuous serving: pieces' entries start here
+    }
+}
+
+// Enqueues one bounce iteration: closest-hit trace, any-hit trace (on the
+// second stream when they overlap: it depends only on the previous shade,
+// like the closest-hit one), shade.  With `spans`, each launch is bracketed
+// by timing events.
+int enqueue_iter(yn_ctx* c, const LoopCfg& L, const IterArgs& A, const DevCapture& cap, std::vector<Span>* spans)
+{
+    hipStream_t s = c->stream;
+    hipStream_t s2 = L.overlap ? c->stream2 : s;
+    auto ev = [&]() -> hipEvent_t { return c->ev[c->ev_used++ % kEvPool]; };
+    if (L.overlap) {
+        HIPCK(c, hipEventRecord(c->fork_ev, s));
+        HIPCK(c, hipStreamWaitEvent(s2, c->fork_ev, 0));
+    }
+    hipEvent_t e0 = spans ? ev() : nullptr, e1 = spans ? ev() : nullptr;
+    if (spans) hipEventRecord(e0, s);
+    launch_trace(A.te, false, L.width, L.max_depth, L.block, L.grid, L.count, c->cfg.lds_stack, s);
+    if (spans) { hipEventRecord(e1, s); spans->push_back({e0, e1, 0}); }
+    hipEvent_t e2 = spans ? ev() : nullptr, e3 = spans ? ev() : nullptr;
+    if (spans) hipEventRecord(e2, s2);
+    launch_trace(A.ts, true, L.width, L.max_depth, L.block, L.grid, L.count, c->cfg.lds_stack, s2);
+    if (spans) { hipEventRecord(e3, s2); spans->push_back({e2, e3, 1}); }
+    if (L.overlap) {
+        HIPCK(c, hipEventRecord(c->join_ev, s2));
+        HIPCK(c, hipStreamWaitEvent(s, c->join_ev, 0));
+    }
+    hipEvent_t e4 = spans ? ev() : nullptr, e5 = spans ? ev() : nullptr;
+    if (spans) hipEventRecord(e4, s);
+    if (L.bdpt) launch_shade_bdpt(A.ba, cap, L.shade_grid, s);
+    else launch_shade_pt(A.sa, cap, L.shade_grid, c->cfg.shade_waves, s);
+    if (spans) { hipEventRecord(e5, s); spans->push_back({e4, e5, 2}); }
+    return 0;
+}
+
 // Runs one batch of primary-ray regions to completion; samples land in
 // samples[slot][pixel] of each region's job (the sample buffer is shared:
 // job slots are laid out by the caller so regions do not overlap).
@@ -344,7 +550,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     std::vector<GenSeg> segs;
     int64_t total = 0;
     for (const Region& r : regs) {
-        GenSeg g{r.job, r.slot, r.rand, r.x0, r.y0, r.w, r.h, (int32_t)total};
+        GenSeg g{r.job, r.slot, r.rand, r.x0, r.y0, r.w, r.h, (int32_t)total, 0};
         total += (int64_t)r.w * r.h;
         segs.push_back(g);
     }
@@ -371,14 +577,15 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     auto q = [&](int k) { return &c->ctr->q[k][0]; };
     // counters (q(k) = q[k][0]): q[0],q[1] extension queue counts, q[2],q[3] shadow queue
     // counts, q[7] stack overflow flag, q[8],q[9] path-list counts
-    // (bidirectional mode), q[10] free slots (regeneration); work[0], work[1]
+    // (bidirectional mode), q[10] free slots (regeneration), q[11] where a
+    // regeneration's rays start in their queue; work[0], work[1]
     // the chunk counters of the extend / shadow traces, per queue slice.
     for (int k = 0; k < 2; ++k) {
         c->E[k].count = q(k);
         c->Sh[k].count = q(2 + k);
         c->ShB[k].count = q(2 + k);
     }
-    RayQueue* SQ = bdpt ? c->ShB : c->Sh;
+    c->ev_used = 0;
 
     GenArgs ga{};
     ga.scenes = scenes;
@@ -397,140 +604,15 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         launch_generate(ga, cap, s);
     }
 
-    // single scene fast path
-    int single = -1;
-    {
-        int sc0 = c->jobs[regs[0].job].scene;
-        bool same = true;
-        for (const Region& r : regs) same &= c->jobs[r.job].scene == sc0;
-        if (same) single = sc0;
-    }
-    int max_depth = 0, width = 2;
-    for (const Region& r : regs) {
-        const SceneRec& sr = c->scenes[c->jobs[r.job].scene];
-        max_depth = std::max(max_depth, sr.info.max_depth);
-        width = sr.info.width;   // all scenes of a context share the width
-    }
-    int block = c->cfg.trace_block;
-    // default: the trace blocks that fit at once (4 SIMDs per CU)
-    int grid = c->cfg.trace_grid > 0 ? c->cfg.trace_grid
-                                     : c->n_cu * std::max(1, trace_resident_waves(c->cfg.lds_stack) * 256 / block);
-    // persistent shade grids: the blocks that fit at once (4 SIMDs per CU;
-    // the bidirectional shade runs 3 waves per SIMD in blocks of 256)
-    int shade_grid = bdpt ? c->n_cu * 3 : std::max(1, c->n_cu * c->cfg.shade_waves * 256 / kShadeBlockPT);
-    const bool overlap = c->cfg.trace_overlap != 0;
-    // overlapping traces get disjoint halves of the stack overflow area
-    if (int r = ensure_overflow(c, max_depth - trace_lds_stack(c->cfg.lds_stack), (int64_t)grid * block * (overlap ? 2 : 1)))
-        return r;
-    bool count = stat_job.counting != 0;
-
-    int ev_base = 0;
-    c->ev_used = 0;
-    auto ev = [&]() -> hipEvent_t { return c->ev[(ev_base + c->ev_used++) % kEvPool]; };
-    struct Span { hipEvent_t a, b; int kind; };
+    std::vector<int> scene_ids;
+    for (const Region& r : regs) scene_ids.push_back(c->jobs[r.job].scene);
+    LoopCfg L;
+    if (int r = loop_cfg(c, bdpt, scene_ids, stat_job.counting != 0, L)) return r;
+    L.free_list = regen ? c->free_list : nullptr;
     std::vector<Span> spans;
-
-    // One bounce iteration's launches; every argument follows from the
-    // iteration's parity (queues ping-pong), so a group of 4 iterations is the
-    // same sequence of launches each time and may be replayed from a graph.
-    struct IterArgs { TraceArgs te, ts; ShadeArgs sa; BdptShadeArgs ba; };
-    auto make_iter = [&](int cur, IterArgs& A) {
-        std::memset(&A, 0, sizeof A);
-        const int nxt = cur ^ 1;
-        TraceArgs& ta = A.te;
-        ta.scenes = scenes;
-        ta.jobs = c->d_jobs;
-        ta.path_job = c->P.job;
-        ta.single_scene = single;
-        ta.chunk = c->cfg.trace_chunk;
-        ta.ctr = c->ctr;
-        ta.ovf_ref = c->ovf_ref;
-        ta.ovf_t = c->ovf_t;
-        ta.ovf_depth = c->ovf_depth;
-        // extension rays
-        ta.q = c->E[cur];
-        ta.hit_out = c->P.hit;
-        ta.work = &c->ctr->work[0][0][0];
-        ta.n_slices = c->cfg.trace_xcd;
-        ta.refill = c->cfg.trace_refill > 0 ? c->cfg.trace_refill : 0;
-        ta.zero0 = c->E[nxt].count;
-        ta.zero1 = SQ[nxt].count;
-        // shadow rays
-        TraceArgs& ts = A.ts;
-        ts = ta;
-        ts.q = SQ[cur];
-        ts.hit_out = nullptr;
-        ts.occ_out = bdpt ? c->B.occ : c->P.occ;
-        ts.id_shift = bdpt ? c->B.shift : 0;
-        ts.work = &c->ctr->work[1][0][0];
-        ts.zero0 = bdpt ? q(8 + nxt) : nullptr;    // the next path list
-        ts.zero1 = nullptr;
-        if (overlap) {          // the second half of the overflow area
-            ts.ovf_ref = c->ovf_ref + (size_t)c->ovf_depth * grid * block;
-            ts.ovf_t = c->ovf_t + (size_t)c->ovf_depth * grid * block;
-        }
-        if (bdpt) {
-            BdptShadeArgs& ba = A.ba;
-            ba.scenes = scenes;
-            ba.jobs = c->d_jobs;
-            ba.P = c->P;
-            ba.B = c->B;
-            ba.list_in = c->lst[cur];
-            ba.list_in_count = q(8 + cur);
-            ba.list_out = c->lst[nxt];
-            ba.list_out_count = q(8 + nxt);
-            ba.ext_out = c->E[nxt];
-            ba.sh_out = c->ShB[nxt];
-            ba.zero = &c->ctr->work[0][0][0];
-            ba.n_zero = 2 * kMaxXcd;
-        } else {
-            ShadeArgs& sa = A.sa;
-            sa.scenes = scenes;
-            sa.jobs = c->d_jobs;
-            sa.P = c->P;
-            sa.ext_in = c->E[cur];
-            sa.sh_in = c->Sh[cur];
-            sa.ext_out = c->E[nxt];
-            sa.sh_out = c->Sh[nxt];
-            sa.zero = &c->ctr->work[0][0][0];
-            sa.n_zero = 2 * kMaxXcd;
-            sa.ctr = c->ctr;
-            sa.free_list = regen ? c->free_list : nullptr;
-            sa.free_count = q(10);
-        }
-    };
-    if (max_depth > trace_lds_stack(c->cfg.lds_stack) + c->ovf_depth)
-        return fail(c, YN_ERR_INVALID, "BVH too deep for the traversal stack");
-    auto enqueue_iter = [&](const IterArgs& A, bool timed) -> int {
-        // the any-hit trace depends only on the previous shade, like the
-        // closest-hit one: with overlap it runs on the second stream
-        hipStream_t s2 = overlap ? c->stream2 : s;
-        if (overlap) {
-            HIPCK(c, hipEventRecord(c->fork_ev, s));
-            HIPCK(c, hipStreamWaitEvent(s2, c->fork_ev, 0));
-        }
-        hipEvent_t e0 = timed ? ev() : nullptr, e1 = timed ? ev() : nullptr;
-        if (timed) hipEventRecord(e0, s);
-        launch_trace(A.te, false, width, max_depth, block, grid, count, c->cfg.lds_stack, s);
-        if (timed) { hipEventRecord(e1, s); spans.push_back({e0, e1, 0}); }
-        hipEvent_t e2 = timed ? ev() : nullptr, e3 = timed ? ev() : nullptr;
-        if (timed) hipEventRecord(e2, s2);
-        launch_trace(A.ts, true, width, max_depth, block, grid, count, c->cfg.lds_stack, s2);
-        if (timed) { hipEventRecord(e3, s2); spans.push_back({e2, e3, 1}); }
-        if (overlap) {
-            HIPCK(c, hipEventRecord(c->join_ev, s2));
-            HIPCK(c, hipStreamWaitEvent(s, c->join_ev, 0));
-        }
-        hipEvent_t e4 = timed ? ev() : nullptr, e5 = timed ? ev() : nullptr;
-        if (timed) hipEventRecord(e4, s);
-        if (bdpt) launch_shade_bdpt(A.ba, cap, shade_grid, s);
-        else launch_shade_pt(A.sa, cap, shade_grid, c->cfg.shade_waves, s);
-        if (timed) { hipEventRecord(e5, s); spans.push_back({e4, e5, 2}); }
-        return 0;
-    };
     IterArgs iters[2];
-    make_iter(0, iters[0]);
-    make_iter(1, iters[1]);
+    make_iter(c, L, 0, iters[0]);
+    make_iter(c, L, 1, iters[1]);
     const int group = 4;                 // iterations between host checks (even)
     // Graph replay of a group: the launches are captured once and replayed
     // while their arguments stay byte-identical (same buffers, scenes, launch
@@ -541,15 +623,15 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     if (use_graph) {
         std::vector<uint8_t> key(sizeof iters);
         std::memcpy(key.data(), iters, sizeof iters);
-        const int32_t shape[] = {bdpt, width, max_depth, block, grid, shade_grid, count, c->cfg.lds_stack,
-                                 c->cfg.shade_waves, overlap, group};
+        const int32_t shape[] = {L.bdpt, L.width, L.max_depth, L.block, L.grid, L.shade_grid, L.count,
+                                 c->cfg.lds_stack, c->cfg.shade_waves, L.overlap, group};
         key.insert(key.end(), (const uint8_t*)shape, (const uint8_t*)shape + sizeof shape);
         if (!c->gexec || key != c->gkey) {
             if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
             c->gkey.clear();
             HIPCK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int r = 0;
-            for (int g = 0; g < group && !r; ++g) r = enqueue_iter(iters[g & 1], false);
+            for (int g = 0; g < group && !r; ++g) r = enqueue_iter(c, L, iters[g & 1], cap, nullptr);
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamEndCapture(s, &graph);
             if (!r && e == hipSuccess) e = hipGraphInstantiateWithFlags(&c->gexec, graph, 0);
@@ -568,7 +650,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         } else {
             for (int g = 0; g < group; ++g, ++it) {
                 bool timed = (int)spans.size() + 3 < kEvPool / 2;
-                if (int r = enqueue_iter(iters[it & 1], timed)) return r;
+                if (int r = enqueue_iter(c, L, iters[it & 1], cap, timed ? &spans : nullptr)) return r;
             }
         }
         HIPCK(c, hipGetLastError());
@@ -591,14 +673,16 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
                 HIPCK(c, hipcub::DeviceRadixSort::SortKeys(c->sort_tmp, tb, c->free_list, c->free_alt, (int)f, 0,
                                                            c->slot_bits, s));
                 std::swap(c->free_list, c->free_alt);
-                make_iter(0, iters[0]);      // the shade appends to the other buffer now
-                make_iter(1, iters[1]);
+                L.free_list = c->free_list;  // the shade appends to the other buffer now
+                make_iter(c, L, 0, iters[0]);
+                make_iter(c, L, 1, iters[1]);
                 GenArgs gr = ga;
                 gr.item0 = (int32_t)issued;
                 gr.n_paths = (int32_t)std::min<int64_t>((int64_t)f, need);
                 gr.free_list = c->free_list;
                 gr.free_top = f;
                 gr.free_count = q(10);
+                gr.queue_base = q(11);
                 gr.ext_out = c->E[nxt];
                 launch_generate(gr, cap, s);
                 issued += gr.n_paths;
@@ -607,13 +691,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         }
         if (empty) break;
     }
-    for (const Span& sp : spans) {
-        float ms = 0.0f;
-        hipEventElapsedTime(&ms, sp.a, sp.b);
-        if (sp.kind == 0) stat_job.st.ms_trace_ext += ms;
-        else if (sp.kind == 1) stat_job.st.ms_trace_shadow += ms;
-        else stat_job.st.ms_shade += ms;
-    }
+    add_spans(spans, stat_job.st);
     stat_job.st.iterations += (uint64_t)it;
     stat_job.st.trace_launches_ext += (uint64_t)it;
     stat_job.st.trace_launches_shadow += (uint64_t)it;
@@ -635,6 +713,8 @@ int upload_job_table(yn_ctx* c, int jid, float4* samples)
     return 0;
 }
 
+int stream_drain(yn_ctx* c);
+
 // Render n frames of rectangle [x0,x0+w) x [y0,y0+h) into `accum`, frames
 // taking seeds from `seeds`; first frame resets when reset_first.
 int render_rect(yn_ctx* c, int jid, float4* accum, int x0, int y0, int w, int h, const std::vector<uint32_t>& seeds,
@@ -643,6 +723,9 @@ int render_rect(yn_ctx* c, int jid, float4* accum, int x0, int y0, int w, int h,
     JobRec& J = c->jobs[jid];
     int64_t npix = (int64_t)w * h;
     if (npix == 0 || seeds.empty()) return 0;
+    // the continuous loop shares the wavefront buffers: it finishes first
+    if (c->strm.active)
+        if (int r = stream_drain(c)) return r;
     int64_t full = (int64_t)J.W * J.H;
     int64_t pcap = c->cap_paths;
     if (J.p.mode == YN_MODE_BDPT) {
@@ -760,6 +843,7 @@ int serve_round(yn_ctx* c, int* frames_out)
             if (r) return r;
             HIPCK(c, hipStreamSynchronize(c->stream));
             J.frame += 1;
+            J.planned = (int64_t)J.frame;
             J.st.frames += 1;
             J.st.samples += (uint64_t)J.W * J.H;
             if ((int64_t)J.frame >= J.target) J.t_done = ms_since(c);
@@ -818,10 +902,324 @@ int serve_round(yn_ctx* c, int* frames_out)
     for (const Plan& pl : plan) {
         JobRec& J = c->jobs[pl.jid];
         J.frame += (uint64_t)pl.k;
+        J.planned = (int64_t)J.frame;
         J.st.frames += (uint64_t)pl.k;
         J.st.samples += (uint64_t)pl.k * J.W * J.H;
         if ((int64_t)J.frame >= J.target) J.t_done = now;
         *frames_out += pl.k;
+    }
+    return 0;
+}
+
+// ---- continuous serving (path tracing) ----------------------------------
+
+int64_t stream_cap(const yn_ctx* c)
+{
+    return c->batch_paths > 0 ? std::min(c->batch_paths, c->cap_paths) : c->cap_paths;
+}
+
+// First fit in the samples buffer's free ranges; -1 when nothing fits.
+int64_t samples_alloc(StreamState& S, int64_t n)
+{
+    for (auto it = S.sfree.begin(); it != S.sfree.end(); ++it) {
+        if (it->second < n) continue;
+        const int64_t off = it->first, len = it->second;
+        S.sfree.erase(it);
+        if (len > n) S.sfree[off + n] = len - n;
+        return off;
+    }
+    return -1;
+}
+
+void samples_free(StreamState& S, int64_t off, int64_t n)
+{
+    auto nx = S.sfree.lower_bound(off);
+    if (nx != S.sfree.end() && off + n == nx->first) { n += nx->second; nx = S.sfree.erase(nx); }
+    if (nx != S.sfree.begin()) {
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == off) { pv->second += n; return; }
+    }
+    S.sfree[off] = n;
+}
+
+// Jobs the continuous loop renders: path tracing, a frame that fits the
+// samples buffer.  The others are served in rounds.
+bool stream_takes(const yn_ctx* c, const JobRec& J)
+{
+    return J.p.mode == YN_MODE_PT && (int64_t)J.W * J.H <= c->sample_cap;
+}
+
+int64_t stream_pending(const yn_ctx* c)
+{
+    int64_t n = 0;
+    for (int i : c->strm.issue) n += c->strm.pieces[i].items - c->strm.pieces[i].issued;
+    return n;
+}
+
+bool stream_has_pieces(const yn_ctx* c)
+{
+    for (const Piece& P : c->strm.pieces) if (P.live) return true;
+    return false;
+}
+
+// An empty wavefront: every slot of the stream's capacity is back on the
+// free list and no piece is waiting.
+int stream_start(yn_ctx* c)
+{
+    StreamState& S = c->strm;
+    hipStream_t s = c->stream;
+    S.active = true;
+    S.it = 0;
+    S.cap = stream_cap(c);
+    S.issue.clear();
+    for (Piece& P : S.pieces) P = Piece();
+    S.sfree.clear();
+    S.sfree[0] = c->sample_cap;
+    S.free_count = (uint32_t)S.cap;
+    S.free_sorted = true;
+    for (auto& v : S.last) v = 0;
+    for (int k = 0; k < 2; ++k) {
+        c->E[k].count = &c->ctr->q[k][0];
+        c->Sh[k].count = &c->ctr->q[2 + k][0];
+    }
+    c->ev_used = 0;
+    HIPCK(c, hipMemsetAsync(c->ctr, 0, sizeof(DevCounters), s));
+    launch_init_free(c->free_list, (int)S.cap, &c->ctr->q[10][0], s);
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+// Cuts admitted path-tracing jobs (oldest first) into pieces until about
+// one capacity of work items waits to enter the wavefront, or the samples
+// buffer or the piece table is full.  A piece holds up to a sixteenth of the
+// capacity in items (at least one frame).
+int stream_plan(yn_ctx* c)
+{
+    StreamState& S = c->strm;
+    int64_t pending = stream_pending(c);
+    if (pending >= S.cap) return 0;
+    std::vector<int> order;
+    for (int i = 0; i < (int)c->jobs.size(); ++i) {
+        const JobRec& J = c->jobs[i];
+        if (J.live && J.served && J.admitted && stream_takes(c, J) && J.planned < J.target) order.push_back(i);
+    }
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return c->jobs[a].seq < c->jobs[b].seq; });
+    const int64_t piece_items = std::max<int64_t>(S.cap / 16, 1);
+    int free_piece = 0;
+    for (int jid : order) {
+        JobRec& J = c->jobs[jid];
+        const int64_t npix = (int64_t)J.W * J.H;
+        while (pending < S.cap && J.planned < J.target) {
+            const int64_t k = std::min<int64_t>(J.target - J.planned, std::max<int64_t>(1, piece_items / npix));
+            const int64_t items = k * npix;
+            while (free_piece < kMaxPieces && S.pieces[free_piece].live) ++free_piece;
+            if (free_piece >= kMaxPieces) { c->serve_rec.st.plan_stalls++; return 0; }
+            const int64_t off = samples_alloc(S, items);
+            if (off < 0) { c->serve_rec.st.plan_stalls++; return 0; }
+            Piece& P = S.pieces[free_piece];
+            P = Piece();
+            P.live = true;
+            P.job = jid;
+            P.scene = J.scene;
+            P.f0 = J.planned;
+            P.k = (int32_t)k;
+            P.items = items;
+            P.soff = off;
+            for (int64_t f = 0; f < k; ++f) P.seeds.push_back((uint32_t)J.engine());
+            J.planned += k;
+            DevJob d = make_devjob(J);
+            d.samples = c->samples + off;
+            HIPCK(c, hipMemcpyAsync(c->d_jobs + kMaxJobs + free_piece, &d, sizeof(DevJob), hipMemcpyHostToDevice,
+                                    c->stream));
+            HIPCK(c, hipMemsetAsync(c->done_ctr + free_piece, 0, sizeof(uint32_t), c->stream));
+            S.issue.push_back(free_piece);
+            pending += items;
+            if (J.t_start < 0) J.t_start = ms_since(c);
+        }
+    }
+    return 0;
+}
+
+// Turns waiting work items into paths in the slots of ended paths: once at
+// least half the capacity is free (or all the waiting work fits, or the
+// wavefront is empty).  Path state is indexed by slot, so new paths whose
+// slots lie far apart cost the shade pass scattered accesses: refilling at
+// half the capacity (27 refills for the 64-job mix) instead of an eighth (45)
+// cut its shade time by 7%.  The new paths' rays are appended to the queue
+// the next iteration traces.
+int stream_refill(yn_ctx* c)
+{
+    StreamState& S = c->strm;
+    const int64_t pending = stream_pending(c);
+    const int64_t f = S.free_count;
+    if (pending == 0 || f == 0) return 0;
+    const bool empty = f >= S.cap;
+    if (!empty && f < std::min<int64_t>(pending, std::max<int64_t>(S.cap / 2, 1))) return 0;
+    const int64_t n = std::min<int64_t>(f, pending);
+    hipStream_t s = c->stream;
+    if (!S.free_sorted && f > 1) {
+        // slots come back in the order paths ended; sorted, new paths take
+        // ascending slots and their state stays grouped
+        size_t tb = c->sort_tmp_bytes;
+        HIPCK(c, hipcub::DeviceRadixSort::SortKeys(c->sort_tmp, tb, c->free_list, c->free_alt, (int)f, 0,
+                                                   c->slot_bits, s));
+        std::swap(c->free_list, c->free_alt);
+    }
+    // segments: items of the waiting pieces in order, frame by frame
+    S.segs.clear();
+    int64_t left = n, base = 0;
+    while (left > 0 && !S.issue.empty()) {
+        Piece& P = S.pieces[S.issue.front()];
+        const JobRec& J = c->jobs[P.job];
+        const int64_t npix = (int64_t)J.W * J.H;
+        const int64_t take = std::min<int64_t>(left, P.items - P.issued);
+        const int64_t a = P.issued, b = a + take;
+        for (int64_t fi = a / npix; fi * npix < b; ++fi) {
+            const int64_t lo = std::max<int64_t>(a, fi * npix), hi = std::min<int64_t>(b, (fi + 1) * npix);
+            S.segs.push_back(GenSeg{kMaxJobs + S.issue.front(), (int32_t)fi, P.seeds[fi], 0, 0, J.W, J.H,
+                                    (int32_t)base, (int32_t)(lo - fi * npix)});
+            base += hi - lo;
+        }
+        P.issued = b;
+        left -= take;
+        if (P.issued == P.items) S.issue.pop_front();
+    }
+    if ((int)S.segs.size() > c->seg_cap) {
+        if (c->d_segs) hipFree(c->d_segs);
+        c->d_segs = nullptr;
+        c->seg_cap = std::max<int>((int)S.segs.size(), 256);
+        if (int r = dalloc(c, &c->d_segs, (size_t)c->seg_cap)) return r;
+    }
+    HIPCK(c, hipMemcpyAsync(c->d_segs, S.segs.data(), sizeof(GenSeg) * S.segs.size(), hipMemcpyHostToDevice, s));
+    GenArgs g{};
+    g.scenes = c->d_scenes;
+    g.jobs = c->d_jobs;
+    g.P = c->P;
+    g.ext_out = c->E[S.it & 1];
+    g.segs = c->d_segs;
+    g.n_segs = (int32_t)S.segs.size();
+    g.n_paths = (int32_t)n;
+    g.item0 = 0;
+    g.free_list = c->free_list;
+    g.free_top = (uint32_t)f;
+    g.free_count = &c->ctr->q[10][0];
+    g.queue_base = &c->ctr->q[11][0];
+    launch_generate(g, DevCapture{}, s);
+    HIPCK(c, hipGetLastError());
+    c->serve_rec.st.refills++;
+    S.free_count = (uint32_t)(f - n);
+    S.free_sorted = true;         // what is left is the sorted list's lower part
+    return 0;
+}
+
+// Folds finished pieces into their jobs' images, each job's in frame order,
+// and gives their samples ranges and table entries back.
+int stream_fold(yn_ctx* c)
+{
+    StreamState& S = c->strm;
+    for (int i = 0; i < kMaxPieces; ++i) {
+        Piece& P = S.pieces[i];
+        if (P.live && !P.done && P.issued == P.items && c->h_done[i] >= (uint32_t)P.items) P.done = true;
+    }
+    const double now = ms_since(c);
+    for (bool progress = true; progress;) {
+        progress = false;
+        for (int i = 0; i < kMaxPieces; ++i) {
+            Piece& P = S.pieces[i];
+            if (!P.live || !P.done) continue;
+            if (!P.orphan) {
+                JobRec& J = c->jobs[P.job];
+                if ((int64_t)J.frame != P.f0) continue;      // an earlier piece of the job is not folded yet
+                ResolveArgs ra{c->d_jobs, kMaxJobs + i, 0, 0, J.W, J.H, P.k, P.f0 == 0 ? 1 : 0, J.accum};
+                launch_resolve(ra, c->stream);
+                J.frame += (uint64_t)P.k;
+                J.st.frames += (uint64_t)P.k;
+                J.st.samples += (uint64_t)P.items;
+                if ((int64_t)J.frame >= J.target) J.t_done = now;
+            }
+            samples_free(S, P.soff, P.items);
+            P = Piece();
+            progress = true;
+        }
+    }
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+// One group of four bounce iterations of the persistent wavefront, with
+// admission, planning and refill before it and folding after it.  `plan`
+// false: no new pieces (the loop drains).  *ran: whether any path was in
+// flight.
+int stream_group(yn_ctx* c, bool plan, bool* ran)
+{
+    StreamState& S = c->strm;
+    *ran = false;
+    const auto h0 = std::chrono::steady_clock::now();
+    if (!S.active)
+        if (int r = stream_start(c)) return r;
+    if (int r = admit_jobs(c)) return r;
+    if (plan)
+        if (int r = stream_plan(c)) return r;
+    if (int r = stream_refill(c)) return r;
+    if ((int64_t)S.free_count >= S.cap) {
+        // nothing in flight: every issued path has ended
+        if (int r = stream_fold(c)) return r;
+        if (S.issue.empty() && !stream_has_pieces(c)) S.active = false;
+        return 0;
+    }
+    std::vector<int> scene_ids;
+    for (const Piece& P : S.pieces)
+        if (P.live) scene_ids.push_back(P.scene);
+    LoopCfg L;
+    if (int r = loop_cfg(c, false, scene_ids, false, L)) return r;
+    L.free_list = c->free_list;
+    L.done_ctr = c->done_ctr;
+    IterArgs iters[2];
+    make_iter(c, L, 0, iters[0]);
+    make_iter(c, L, 1, iters[1]);
+    std::vector<Span> spans;
+    c->ev_used = 0;
+    c->serve_rec.st.groups++;
+    c->serve_rec.st.paths_in_flight += (uint64_t)(S.cap - (int64_t)S.free_count);
+    c->serve_rec.st.ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    for (int g = 0; g < 4; ++g, ++S.it)
+        if (int r = enqueue_iter(c, L, iters[S.it & 1], DevCapture{}, &spans)) return r;
+    hipStream_t s = c->stream;
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, kCtrHostBytes, hipMemcpyDeviceToHost, s));
+    HIPCK(c, hipMemcpyAsync(c->h_done, c->done_ctr, sizeof(uint32_t) * kMaxPieces, hipMemcpyDeviceToHost, s));
+    HIPCK(c, hipStreamSynchronize(s));
+    *ran = true;
+    if (c->h_ctr->q[7][0]) return fail(c, YN_ERR_INVALID, "traversal stack overflow");
+    yn_stats& st = c->serve_rec.st;
+    add_spans(spans, st);
+    const unsigned long long now[6] = {c->h_ctr->rays_ext, c->h_ctr->rays_shadow, c->h_ctr->nodes_ext,
+                                       c->h_ctr->tris_ext, c->h_ctr->nodes_shadow, c->h_ctr->tris_shadow};
+    st.rays_ext += now[0] - S.last[0];
+    st.rays_shadow += now[1] - S.last[1];
+    st.nodes_ext += now[2] - S.last[2];
+    st.tris_ext += now[3] - S.last[3];
+    st.nodes_shadow += now[4] - S.last[4];
+    st.tris_shadow += now[5] - S.last[5];
+    for (int k = 0; k < 6; ++k) S.last[k] = now[k];
+    st.iterations += 4;
+    st.trace_launches_ext += 4;
+    st.trace_launches_shadow += 4;
+    S.free_count = c->h_ctr->q[10][0];
+    S.free_sorted = false;
+    return stream_fold(c);
+}
+
+// Runs the continuous loop without new pieces until every issued path has
+// ended and every piece is folded: other entry points use the same
+// wavefront buffers.
+int stream_drain(yn_ctx* c)
+{
+    for (int guard = 0; c->strm.active; ++guard) {
+        if (guard > 1000000) return fail(c, YN_ERR_HIP, "continuous loop did not drain");
+        // items not yet issued stay with their pieces; finish them too
+        bool ran = false;
+        if (int r = stream_group(c, false, &ran)) return r;
     }
     return 0;
 }
@@ -946,6 +1344,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (k.trace_chunk <= 0) k.trace_chunk = 64;
     if (k.hoist_frac == 0.0f) k.hoist_frac = 0.01f;
     if (k.bvh_build == 0) k.bvh_build = 1;
+    if (k.serve_mode != 0 && k.serve_mode != 1) return fail(nullptr, YN_ERR_INVALID, "serve_mode must be 0 or 1");
     if (k.bvh_build != 1 && k.bvh_build != 2) return fail(nullptr, YN_ERR_INVALID, "bvh_build must be 1 or 2");
     if (k.leaf_max <= 0) k.leaf_max = k.bvh_build == 1 ? 1 : 2;
     if (k.leaf_max > 10) k.leaf_max = 10;
@@ -996,15 +1395,19 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     }
     c->scenes.resize(kMaxScenes);
     c->jobs.resize(kMaxJobs);
-    if ((r = dalloc(c, &c->d_scenes, kMaxScenes)) || (r = dalloc(c, &c->d_jobs, kMaxJobs)) || (r = dalloc(c, &c->ctr, 1))) {
+    if ((r = dalloc(c, &c->d_scenes, kMaxScenes)) || (r = dalloc(c, &c->d_jobs, kMaxJobs + kMaxPieces)) ||
+        (r = dalloc(c, &c->ctr, 1)) || (r = dalloc(c, &c->done_ctr, kMaxPieces))) {
         g_err = c->err; return bail(r);
     }
     c->allocs.push_back(c->d_scenes);
     c->allocs.push_back(c->d_jobs);
     c->allocs.push_back(c->ctr);
-    if (hipHostMalloc((void**)&c->h_ctr, sizeof(DevCounters), hipHostMallocDefault) != hipSuccess) {
+    c->allocs.push_back(c->done_ctr);
+    if (hipHostMalloc((void**)&c->h_ctr, sizeof(DevCounters), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_done, sizeof(uint32_t) * kMaxPieces, hipHostMallocDefault) != hipSuccess) {
         g_err = "hipHostMalloc failed"; return bail(YN_ERR_HIP);
     }
+    c->strm.pieces.resize(kMaxPieces);
     if ((r = alloc_wavefront(c, k.max_paths))) { g_err = c->err; return bail(r); }
     *out = c;
     return 0;
@@ -1020,6 +1423,7 @@ void yn_destroy(yn_ctx* c)
     if (c->ovf_ref) hipFree(c->ovf_ref);
     if (c->ovf_t) hipFree(c->ovf_t);
     if (c->h_ctr) hipHostFree(c->h_ctr);
+    if (c->h_done) hipHostFree(c->h_done);
     for (int i = 0; i < kEvPool; ++i) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->stream2) hipStreamDestroy(c->stream2);
@@ -1064,6 +1468,11 @@ int yn_scene_release(yn_ctx* c, int32_t sid)
     if (check_ctx(c) || check_scene(c, sid)) return YN_ERR_NOT_FOUND;
     for (auto& j : c->jobs)
         if (j.live && j.scene == sid) return fail(c, YN_ERR_INVALID, "scene still used by a job");
+    for (const Piece& P : c->strm.pieces)
+        if (P.live && P.scene == sid) {     // paths of a released job still trace it
+            if (int r = stream_drain(c)) return r;
+            break;
+        }
     HIPCK(c, hipStreamSynchronize(c->stream));
     for (void*& b : c->scenes[sid].bufs) { if (b) hipFree(b); b = nullptr; }
     c->scenes[sid] = SceneRec();
@@ -1253,6 +1662,14 @@ int yn_job_release(yn_ctx* c, int32_t jid)
 {
     if (check_ctx(c) || check_job(c, jid)) return YN_ERR_NOT_FOUND;
     JobRec& J = c->jobs[jid];
+    // pieces of the job in the continuous loop: the paths already issued
+    // finish (their samples are dropped), the rest is never issued
+    StreamState& S = c->strm;
+    for (Piece& P : S.pieces)
+        if (P.live && P.job == jid && !P.orphan) { P.orphan = true; P.items = P.issued; }
+    S.issue.erase(std::remove_if(S.issue.begin(), S.issue.end(),
+                                 [&](int i) { return S.pieces[i].issued == S.pieces[i].items; }),
+                  S.issue.end());
     HIPCK(c, hipStreamSynchronize(c->stream));
     if (J.accum) hipFree(J.accum);
     c->job_bytes -= J.bytes;
@@ -1297,9 +1714,33 @@ int yn_serve(yn_ctx* c, int32_t max_rounds, int32_t* pending)
 {
     if (check_ctx(c)) return YN_ERR_INVALID;
     for (int r = 0; r < max_rounds; ++r) {
-        int frames = 0;
-        if (int e = serve_round(c, &frames)) return e;
-        if (frames == 0) break;
+        if (int e = admit_jobs(c)) return e;
+        // the oldest admitted job with frames left decides what runs next
+        int oldest = -1;
+        for (int i = 0; i < (int)c->jobs.size(); ++i) {
+            const JobRec& J = c->jobs[i];
+            if (J.live && J.served && J.admitted && (int64_t)J.frame < J.target &&
+                (oldest < 0 || J.seq < c->jobs[oldest].seq))
+                oldest = i;
+        }
+        const bool rounds = c->cfg.serve_mode == 1;
+        const bool round_job = oldest >= 0 && (rounds || !stream_takes(c, c->jobs[oldest]));
+        if (round_job || rounds) {
+            // a scheduler round; the continuous loop finishes its pieces first
+            if (c->strm.active) {
+                bool ran = false;
+                if (int e = stream_group(c, false, &ran)) return e;
+                continue;
+            }
+            int frames = 0;
+            if (int e = serve_round(c, &frames)) return e;
+            if (frames == 0) break;
+            continue;
+        }
+        if (oldest < 0 && !c->strm.active) break;
+        bool ran = false;
+        if (int e = stream_group(c, true, &ran)) return e;
+        if (!ran && oldest < 0 && !c->strm.active) break;
     }
     if (pending) {
         int n = 0;
@@ -1356,6 +1797,8 @@ int yn_autotune(yn_ctx* c, int32_t sid, const yn_cam* cam, int32_t W, int32_t H,
     for (const JobRec& J : c->jobs)
         if (J.live && J.served && (int64_t)J.frame < J.target)
             return fail(c, YN_ERR_INVALID, "autotune while served jobs are pending");
+    if (c->strm.active)
+        if (int r = stream_drain(c)) return r;
     int tried = 0;
     int err = 0;
     auto measure = [&]() -> double {
@@ -1550,6 +1993,8 @@ int yn_trace(yn_ctx* c, int32_t sid, const float* rays, int32_t n, int32_t* tri,
     if (n < 0 || (n > 0 && (!rays || !tri || !light || !t))) return fail(c, YN_ERR_INVALID, "bad trace arguments");
     if (n == 0) return 0;
     if (n > c->cap_paths) return fail(c, YN_ERR_INVALID, "more rays than the path capacity");
+    if (c->strm.active)
+        if (int r = stream_drain(c)) return r;
     hipStream_t s = c->stream;
     std::vector<int> idx[2];
     for (int i = 0; i < n; ++i) idx[rays[8 * (size_t)i + 7] != 0.0f ? 1 : 0].push_back(i);

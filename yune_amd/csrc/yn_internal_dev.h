@@ -31,11 +31,14 @@ struct TraceArgs {
 };
 
 // Segment of primary rays: a rectangle of one job's image at one frame.
+// Its work items are the rectangle's pixels in row order from pixel
+// `first` on (0 unless a frame's pixels are issued in several parts).
 struct GenSeg {
     int32_t job, slot;
     uint32_t rand;              // frame seed (kernel arg 10)
     int32_t x0, y0, w, h;
     int32_t path_base;          // first path index of the segment
+    int32_t first;              // pixel of the rectangle its first item renders
 };
 
 // Threads per block of the path-tracing shade: its queue appends are one
@@ -53,6 +56,8 @@ struct ShadeArgs {
     DevCounters* ctr;
     uint32_t* free_list;        // slots of ended paths (regeneration), or null
     uint32_t* free_count;
+    uint32_t* done_ctr;         // continuous serving: paths ended per job-table entry from
+    int32_t done_base;          //   done_base on (done_ctr[entry - done_base]), or null
 };
 
 // Shade pass of the bidirectional tracer: advances the paths of `list_in`
@@ -84,12 +89,14 @@ struct GenArgs {
     uint32_t* list_out;         // bidirectional mode: every path starts listed
     uint32_t* list_count;
     // regeneration: work items [item0, item0 + n_paths) take the slots
-    // free_list[free_top - 1 - i] and append their rays to ext_out; without
-    // a free list, item i takes slot i and ray position i
+    // free_list[free_top - n_paths + i] (the list's top n_paths entries, in
+    // list order) and append their rays to ext_out; without a free list,
+    // item i takes slot i and ray position i
     int32_t item0;
     const uint32_t* free_list;
     uint32_t free_top;
     uint32_t* free_count;
+    uint32_t* queue_base;       // with a free list: where the new rays start in ext_out
 };
 
 struct ResolveArgs {
@@ -224,6 +231,7 @@ void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, int wa
 void launch_generate_bdpt(const GenArgs& a, const DevCapture& cap, hipStream_t stream);
 void launch_shade_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int grid, hipStream_t stream);
 void launch_resolve(const ResolveArgs& a, hipStream_t stream);
+void launch_init_free(uint32_t* list, int n, uint32_t* count, hipStream_t stream);
 void launch_fmath(int op, const float* x, const float* y, int n, float* out, hipStream_t stream);
 void launch_tonemap(const float4* in, float4* out, int n, int op, hipStream_t stream);
 void launch_prepare(const DevScene* scene, const float* rays, int n, RayQueue q, int32_t* light_out,
