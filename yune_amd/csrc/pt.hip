@@ -58,30 +58,90 @@ struct Out {
     float4 d_p;
 };
 
+// A path's state between two shade steps (the fields of PathRecs).
+struct Rec {
+    int4 m0, m1;
+    f4 hp, hn, rd, tp, direct, indirect, nd, bnc, wi, ls, mis;
+};
+
 __device__ __forceinline__ void write_sample(const DevJob& J, int slot, int pix, f4 c)
 {
     if (any_nan(c)) c = mk(kPinkR, kPinkG, kPinkB, 1.0f);     // udpt.cl:193-194
     J.samples[(size_t)slot * (size_t)(J.W * J.H) + (size_t)pix] = st4(c);
 }
 
-__device__ __forceinline__ Out make_ray(int kind, f4 o, f4 d, float tmax, int p)
+// The ray's w carries the path's job-table entry: the trace kernels look
+// the scene up through it.
+__device__ __forceinline__ Out make_ray(int kind, f4 o, f4 d, float tmax, int job)
 {
     Out r;
     r.kind = kind;
     r.o_t = make_float4(o.x, o.y, o.z, tmax);
-    r.d_p = make_float4(d.x, d.y, d.z, __int_as_float(p));
+    r.d_p = make_float4(d.x, d.y, d.z, __int_as_float(job));
     return r;
 }
 
-// One shade step of path p.  The ray whose result arrived is rebuilt from
+// The record a step reads: what its state uses.  A new path's record holds
+// only m0, m1 and the camera ray's direction.
+__device__ __forceinline__ void load_rec(const PathRecs& I, uint32_t ri, Rec& R)
+{
+    R.m0 = I.m0[ri];
+    R.m1 = I.m1[ri];
+    R.rd = ld(I.rd[ri]);
+    const int code = R.m0.y & ST_MASK;
+    if (code == ST_PRIMARY) return;
+    R.hp = ld(I.hp[ri]);
+    R.hn = ld(I.hn[ri]);
+    R.tp = ld(I.tp[ri]);
+    R.direct = ld(I.direct[ri]);
+    R.indirect = ld(I.indirect[ri]);
+    if (code == ST_BOUNCE) {
+        R.nd = ld(I.nd[ri]);
+        R.bnc = ld(I.bnc[ri]);
+    } else {
+        R.wi = ld(I.wi[ri]);
+        if (code == ST_NEE_MIS) {
+            R.ls = ld(I.ls[ri]);
+            R.mis = ld(I.mis[ri]);
+        }
+    }
+}
+
+// The record beside a ray just made: what the next step's state uses.
+__device__ __forceinline__ void store_rec(const PathRecs& O, uint32_t ri, const Rec& R)
+{
+    O.m0[ri] = R.m0;
+    O.m1[ri] = R.m1;
+    O.hp[ri] = st4(R.hp);
+    O.hn[ri] = st4(R.hn);
+    O.rd[ri] = st4(R.rd);
+    O.tp[ri] = st4(R.tp);
+    O.direct[ri] = st4(R.direct);
+    O.indirect[ri] = st4(R.indirect);
+    const int code = R.m0.y & ST_MASK;
+    if (code == ST_BOUNCE) {
+        O.nd[ri] = st4(R.nd);
+        O.bnc[ri] = st4(R.bnc);
+    } else {
+        O.wi[ri] = st4(R.wi);
+        if (code == ST_NEE_MIS) {
+            O.ls[ri] = st4(R.ls);
+            O.mis[ri] = st4(R.mis);
+        }
+    }
+}
+
+// One shade step of the path whose ray sits at entry qi of the extension
+// queue (ext) or of the shadow queue; R holds its record, updated in place
+// for the ray the step makes.  The ray whose result arrived is rebuilt from
 // the path state, bit for bit as it was made: a camera ray starts at the
 // camera's eye (createRay, udpt.cl:236) along the stored direction rd; a
 // bounce ray at hp + nd * EPSILON along the pending bounce direction nd
 // (udpt.cl:766, 906, 945).
-__device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
+__device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint32_t qi, Rec& R)
 {
-    const PathSoA& P = a.P;
-    const int4 m0 = P.g0[2 * (size_t)p], m1 = P.g0[2 * (size_t)p + 1];
+    load_rec(a.in, ext ? qi : (uint32_t)(a.in.cap - 1 - (int64_t)qi), R);
+    const int4 m0 = R.m0, m1 = R.m1;
     const int jid = m1.y;
     const DevJob& J = a.jobs[jid];
     const yn_render_params& RP = J.p;
@@ -98,23 +158,14 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
     const bool cap_on = cap.prim_tri != nullptr && slot == 0;
 
     int vmat = m0.w;                    // material of the path's current vertex
-    const size_t r4 = 4 * (size_t)p;
-    f4 hp = ld(P.g1[r4]), hn = ld(P.g1[r4 + 1]), rd = ld(P.g1[r4 + 2]);
+    f4& hp = R.hp;
+    f4& hn = R.hn;
+    f4& rd = R.rd;
+    f4& tp = R.tp;
+    f4& direct = R.direct;
+    f4& indirect = R.indirect;
     float nee_prob = __int_as_float(m1.w);
-    bool np_dirty = false;
-    // throughput and radiance sums are read only by the steps that use them
-    // and written back only when changed (the shade pass is bound by this
-    // path-state traffic)
-    f4 tp = mk(0, 0, 0, 0), direct = mk(0, 0, 0, 0), indirect = mk(0, 0, 0, 0);
-    bool acc_loaded = false, tp_dirty = false, dir_dirty = false, ind_dirty = false, v_dirty = false;
-    auto load_acc = [&]() {
-        if (!acc_loaded) {
-            tp = ld(P.g1[r4 + 3]);
-            direct = ld(P.g2[r4]);
-            indirect = ld(P.g2[r4 + 1]);
-            acc_loaded = true;
-        }
-    };
+    bool v_dirty = false;
     int bounce = m0.z;
 
     // NEE scratch
@@ -130,7 +181,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
     f4 const_color = mk(0, 0, 0, 0);
 
     if (code == ST_PRIMARY) {
-        float4 h = P.hit[p];
+        float4 h = a.hit[qi];
         int htri = __float_as_int(h.y);
         if (cap_on) {
             cap.prim_tri[pix] = htri;
@@ -154,7 +205,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
             tp = mk(1, 1, 1, 1);
             direct = mk(0, 0, 0, 0);
             indirect = mk(0, 0, 0, 0);
-            acc_loaded = tp_dirty = dir_dirty = ind_dirty = v_dirty = true;
+            v_dirty = true;
             bounce = 0;
             Mat m = load_mat(S.mats, vmat);
             if (!m.spec) { st |= F_DIRECT; step = S_NEE_START; }
@@ -162,20 +213,19 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
         }
     } else {
         if (code == ST_NEE_SH) {
-            occ = P.occ[p];
+            occ = a.occ[qi];
             step = S_NEE_SHADOW;
         } else if (code == ST_NEE_MIS) {
-            occ = P.occ[p];
+            occ = a.occ[qi];
             step = S_NEE_MIS;
         } else {   // ST_BOUNCE: extension result (udpt.cl:488-529)
-            float4 h = P.hit[p];
+            float4 h = a.hit[qi];
             int htri = __float_as_int(h.y);
             Mat m = load_mat(S.mats, vmat);
-            f4 nd = ld(P.g2[r4 + 2]);
-            float4 bnc = P.g2[r4 + 3];
-            load_acc();
+            f4 nd = R.nd;
+            float4 bnc = st4(R.bnc);
             if (htri < 0) {
-                if ((st & F_LCAND) && m.spec) { indirect = indirect + tp * ld(L.ke); ind_dirty = true; }
+                if ((st & F_LCAND) && m.spec) indirect = indirect + tp * ld(L.ke);
                 step = S_FINISH;
             } else {
                 f4 o = hp + nd * kEps;
@@ -184,7 +234,6 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
                 const int nmat = tri_material(S.tri_shade, htri);
                 Mat nm = load_mat(S.mats, nmat);
                 indirect = indirect + tp * nm.ke;
-                ind_dirty = tp_dirty = true;
                 if (m.spec) tp = tp * bnc.z;
                 else
                     tp = ((tp * eval_brdf(m, nd, -rd, hp, hn, (st & F_BGLOSSY) != 0, bnc.y, on, false)) *
@@ -239,18 +288,16 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
             cap.shadow_d[pix] = make_float4(wi.x, wi.y, wi.z, 1.0f);
         }
         if (quad_hit(L, so, wi, len, tq)) { occ = 1; step = S_NEE_SHADOW; break; }
-        P.g3[r4] = make_float4(wi.x, wi.y, wi.z, lpdf);
+        R.wi = mk(wi.x, wi.y, wi.z, lpdf);
         nee_prob = nprob;
-        np_dirty = true;
         st = (st & ~ST_MASK) | ST_NEE_SH;
-        out = make_ray(2, so, wi, len, p);
+        out = make_ray(2, so, wi, len, jid);
         goto steps_done;
     } while (0);
     if (step == S_NEE_SHADOW) do {  // udpt.cl:557-565 and the MIS light weight
         if (code == ST_NEE_SH) {
-            float4 w4 = P.g3[r4];
-            wi = mk(w4.x, w4.y, w4.z, 0.0f);
-            lpdf = w4.w;
+            wi = mk(R.wi.x, R.wi.y, R.wi.z, 0.0f);
+            lpdf = R.wi.w;
             nprob = nee_prob;
             code = ST_DONE;     // consumed
         }
@@ -278,24 +325,22 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
         float tq;
         if (!quad_hit(L, bo, bd, INFINITY, tq)) { nee_res = ls + em; step = S_NEE_DONE; break; }
         st = glossy ? (st | F_NGLOSSY) : (st & ~F_NGLOSSY);
-        P.g3[r4 + 1] = st4(ls);
-        P.g3[r4 + 2] = make_float4(bd.x, bd.y, bd.z, bpdf);
-        P.g3[r4] = make_float4(wi.x, wi.y, wi.z, lpdf);
+        R.ls = ls;
+        R.mis = mk(bd.x, bd.y, bd.z, bpdf);
+        R.wi = mk(wi.x, wi.y, wi.z, lpdf);
         nee_prob = nprob;
-        np_dirty = true;
         st = (st & ~ST_MASK) | ST_NEE_MIS;
-        out = make_ray(2, bo, bd, tq, p);
+        out = make_ray(2, bo, bd, tq, jid);
         goto steps_done;
     } while (0);
     if (step == S_NEE_MIS) do {     // udpt.cl:599-607
         Mat m = load_mat(S.mats, vmat);
         em = m.ke;
-        ls = ld(P.g3[r4 + 1]);
+        ls = R.ls;
         if (occ) { nee_res = ls + em; step = S_NEE_DONE; break; }
-        float4 mb = P.g3[r4 + 2];
-        f4 bd = mk(mb.x, mb.y, mb.z, 0.0f);
-        float bpdf = mb.w;
-        lpdf = P.g3[r4].w;
+        f4 bd = mk(R.mis.x, R.mis.y, R.mis.z, 0.0f);
+        float bpdf = R.mis.w;
+        lpdf = R.wi.w;
         nprob = nee_prob;
         bool glossy = (st & F_NGLOSSY) != 0;
         float mw = power_heuristic(bpdf, lpdf, bpdf);
@@ -306,9 +351,8 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
         break;
     } while (0);
     if (step == S_NEE_DONE) do {
-        load_acc();
-        if (st & F_DIRECT) { direct = nee_res; dir_dirty = true; }
-        else { indirect = indirect + tp * nee_res; ind_dirty = true; }
+        if (st & F_DIRECT) direct = nee_res;
+        else indirect = indirect + tp * nee_res;
         step = S_BOUNCE_SAMPLE;
         break;
     } while (0);
@@ -330,14 +374,13 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, int p)
         st &= ~F_LCAND;
         if (quad_hit(L, no, nd, INFINITY, tq)) { tmax = tq; st |= F_LCAND; }
         st = glossy ? (st | F_BGLOSSY) : (st & ~F_BGLOSSY);
-        P.g2[r4 + 2] = st4(nd);
-        P.g2[r4 + 3] = make_float4(pdf, bprob, ior, 0.0f);
+        R.nd = nd;
+        R.bnc = mk(pdf, bprob, ior, 0.0f);
         st = (st & ~ST_MASK) | ST_BOUNCE;
-        out = make_ray(1, no, nd, tmax, p);
+        out = make_ray(1, no, nd, tmax, jid);
         goto steps_done;
     } while (0);
     if (step == S_FINISH) do {
-        load_acc();
         write_sample(J, slot, pix, direct + indirect);
         st = ST_DONE;
         ended = true;
@@ -351,33 +394,22 @@ steps_done:
     }
     out.done = out.kind == 0;
     out.job = jid;
-
-    P.g0[2 * (size_t)p] = make_int4((int)seed, st, out.kind != 0 && v_dirty ? bounce : m0.z,
-                                    out.kind != 0 && v_dirty ? vmat : m0.w);
-    if (out.kind != 0) {
-        if (np_dirty) P.g0[2 * (size_t)p + 1] = make_int4(m1.x, m1.y, m1.z, __float_as_int(nee_prob));
-        if (v_dirty) {
-            P.g1[r4] = st4(hp);
-            P.g1[r4 + 1] = st4(hn);
-            P.g1[r4 + 2] = st4(rd);
-        }
-        if (tp_dirty) P.g1[r4 + 3] = st4(tp);
-        if (dir_dirty) P.g2[r4] = st4(direct);
-        if (ind_dirty) P.g2[r4 + 1] = st4(indirect);
-    }
+    R.m0 = make_int4((int)seed, st, v_dirty ? bounce : m0.z, v_dirty ? vmat : m0.w);
+    R.m1 = make_int4(m1.x, m1.y, m1.z, __float_as_int(nee_prob));
     return out;
 }
 
 }  // namespace
 
-// WAVES: the occupancy the register allocation is held to, per SIMD (the
-// kernel needs ~133 VGPRs unconstrained, 3 waves; 4 fits in 128 with a few
-// spilled; 5 and 6 spill more and are slower).  Blocks of kShadeBlockPT
-// threads walk the queues together, one atomic per block and queue.
+// WAVES: the occupancy the register allocation is held to, per SIMD.
+// Blocks of kShadeBlockPT threads walk the queues together, one atomic per
+// block and output queue.  A path's record is read beside the ray whose
+// result it consumes and written, once the block knows where the path's
+// next ray goes, beside that ray.
 template <int WAVES>
 __global__ __launch_bounds__(kShadeBlockPT, WAVES) void k_shade_pt(ShadeArgs a, DevCapture cap)
 {
-    __shared__ BlockReserveLds<3, kShadeBlockPT> rsv;
+    __shared__ BlockReserveLds<2, kShadeBlockPT> rsv;
     // continuous serving: ended paths per job-table entry, gathered per block
     // in a small table and added to the device counters once at the end
     // (device-scope atomics on one cache line, one per wave, serialise)
@@ -399,31 +431,35 @@ __global__ __launch_bounds__(kShadeBlockPT, WAVES) void k_shade_pt(ShadeArgs a, 
         }
         atomicAdd(a.done_ctr + (e - a.done_base), n);     // table full: straight to the counter
     };
-    uint32_t* const ctr[3] = {a.ext_out.count, a.sh_out.count, a.free_count};
+    uint32_t* const ctr[2] = {a.ext_out.count, a.sh_out.count};
     int par = 0;
     // the block walks the queues together: its appends are one atomic per queue
     constexpr uint32_t B = kShadeBlockPT;
     for (uint32_t base = blockIdx.x * B; base < total; base += gridDim.x * B, par ^= 1) {
         uint32_t i = base + threadIdx.x;
         Out o;
-        int pid = -1;
+        Rec R;
         if (i < total) {
-            bool ext = i < n_ext;
-            uint32_t k = ext ? i : i - n_ext;
-            pid = __float_as_int((ext ? a.ext_in.d_p[k] : a.sh_in.d_p[k]).w);
-            o = advance(a, cap, pid);
+            const bool ext = i < n_ext;
+            o = advance(a, cap, ext, ext ? i : i - n_ext, R);
         }
-        const bool fr = a.free_list != nullptr && pid >= 0 && o.done;
-        const uint32_t n[3] = {o.kind == 1 ? 1u : 0u, o.kind == 2 ? 1u : 0u, fr ? 1u : 0u};
-        uint32_t at[3];
-        block_reserve<3, kShadeBlockPT>(rsv, par, ctr, n, at);
-        if (o.kind == 1) { a.ext_out.o_t[at[0]] = o.o_t; a.ext_out.d_p[at[0]] = o.d_p; }
-        if (o.kind == 2) { a.sh_out.o_t[at[1]] = o.o_t; a.sh_out.d_p[at[1]] = o.d_p; }
-        if (fr) a.free_list[at[2]] = (uint32_t)pid;     // hand the slot back for regeneration
+        const uint32_t n[2] = {o.kind == 1 ? 1u : 0u, o.kind == 2 ? 1u : 0u};
+        uint32_t at[2];
+        block_reserve<2, kShadeBlockPT>(rsv, par, ctr, n, at);
+        if (o.kind == 1) {
+            a.ext_out.o_t[at[0]] = o.o_t;
+            a.ext_out.d_p[at[0]] = o.d_p;
+            store_rec(a.out, at[0], R);
+        }
+        if (o.kind == 2) {
+            a.sh_out.o_t[at[1]] = o.o_t;
+            a.sh_out.d_p[at[1]] = o.d_p;
+            store_rec(a.out, (uint32_t)(a.out.cap - 1 - (int64_t)at[1]), R);
+        }
         if (a.done_ctr) {
-            // ended paths per job-table entry, one atomic per entry present
-            // in the wave (mostly one: a queue's paths come in runs of one job)
-            const bool dn = pid >= 0 && o.done;
+            // ended paths per job-table entry, one table update per entry
+            // present in the wave (mostly one: a queue's paths come in runs of one job)
+            const bool dn = i < total && o.done;
             unsigned long long m = __ballot(dn);
             while (m) {
                 const int leader = __ffsll((long long)m) - 1;
@@ -442,23 +478,16 @@ __global__ __launch_bounds__(kShadeBlockPT, WAVES) void k_shade_pt(ShadeArgs a, 
 }
 
 // Primary rays (pathtracer entry, udpt.cl:162-189) for a list of segments:
-// work item -> (segment, pixel), into a path slot.
+// work item -> (segment, pixel), a ray at the item's queue position and the
+// new path's record beside it.
 __global__ __launch_bounds__(256) void k_generate(GenArgs a, DevCapture cap)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < a.n_paths;
+    if (i >= a.n_paths) return;
     const int item = a.item0 + i;
-    int p = i;
-    if (a.free_list && live) p = (int)a.free_list[a.free_top - (uint32_t)a.n_paths + (uint32_t)i];
-    // regeneration appends to the queue behind the rays already in it
-    uint32_t qpos = (uint32_t)i;
-    if (a.free_list) {
-        // behind the rays already queued: k_queue_reserve took the count
-        // before this launch and added n_paths to it
-        qpos = *a.queue_base + (uint32_t)i;
-        if (i == 0) *a.free_count = a.free_top - (uint32_t)a.n_paths;
-    }
-    if (!live) return;
+    // behind the rays already queued: k_queue_reserve took the count before
+    // this launch and added n_paths to it
+    const uint32_t qpos = a.queue_base ? *a.queue_base + (uint32_t)i : (uint32_t)i;
     int lo = 0, hi = a.n_segs - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -477,13 +506,12 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a, DevCapture cap)
     float tq, tmax = INFINITY;
     int st = ST_PRIMARY;
     if (quad_hit(S.light, o, d, INFINITY, tq)) { tmax = tq; st |= F_LCAND; }
-    a.P.g0[2 * (size_t)p] = make_int4((int)seed, st, 0, -1);
-    a.P.g0[2 * (size_t)p + 1] = make_int4(y * J.W + x, g.job, g.slot, 0);
-    a.P.job[p] = g.job;                 // read by the trace kernels' scene lookup
-    a.P.g1[4 * (size_t)p + 2] = st4(d);
+    a.rec.m0[qpos] = make_int4((int)seed, st, 0, -1);
+    a.rec.m1[qpos] = make_int4(y * J.W + x, g.job, g.slot, 0);
+    a.rec.rd[qpos] = st4(d);
     a.ext_out.o_t[qpos] = make_float4(o.x, o.y, o.z, tmax);
-    a.ext_out.d_p[qpos] = make_float4(d.x, d.y, d.z, __int_as_float(p));
-    if (!a.free_list && i == 0) *a.ext_out.count = (uint32_t)a.n_paths;
+    a.ext_out.d_p[qpos] = make_float4(d.x, d.y, d.z, __int_as_float(g.job));
+    if (!a.queue_base && i == 0) *a.ext_out.count = (uint32_t)a.n_paths;
     if (cap.prim_tri && g.slot == 0) {
         cap.shadow_occ[y * J.W + x] = -1;
         cap.shadow_d[y * J.W + x] = make_float4(0, 0, 0, 0);
@@ -574,20 +602,6 @@ __global__ __launch_bounds__(256) void k_fmath(int op, const float* x, const flo
     out[i] = op == 0 ? ynf_sinf(x[i]) : op == 1 ? ynf_cosf(x[i]) : ynf_powf(x[i], y[i]);
 }
 
-// Free list holding every slot 0..n-1 (ascending), count n: the empty
-// wavefront of the continuous serving loop.
-__global__ __launch_bounds__(256) void k_init_free(uint32_t* list, int n, uint32_t* count)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) list[i] = (uint32_t)i;
-    if (i == 0) *count = (uint32_t)n;
-}
-
-void launch_init_free(uint32_t* list, int n, uint32_t* count, hipStream_t s)
-{
-    if (n > 0) hipLaunchKernelGGL(k_init_free, dim3((n + 255) / 256), dim3(256), 0, s, list, n, count);
-}
-
 void launch_fmath(int op, const float* x, const float* y, int n, float* out, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_fmath, dim3((n + 255) / 256), dim3(256), 0, s, op, x, y, n, out);
@@ -607,7 +621,7 @@ void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t s)
 {
     int grid = (a.n_paths + 255) / 256;
     if (grid <= 0) return;
-    if (a.free_list)
+    if (a.queue_base)
         hipLaunchKernelGGL(k_queue_reserve, dim3(1), dim3(64), 0, s, a.ext_out.count, (uint32_t)a.n_paths,
                            a.queue_base);
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, a, cap);

@@ -22,7 +22,6 @@
 #include <string>
 #include <vector>
 
-#include <hipcub/hipcub.hpp>
 
 #include "../../include/yune_hip.h"
 #include "yn_internal_dev.h"
@@ -113,8 +112,7 @@ struct StreamState {
     std::vector<Piece> pieces;
     std::deque<int> issue;            // pieces with items left to issue, in order
     std::map<int64_t, int64_t> sfree; // free ranges of the samples buffer: offset -> length
-    uint32_t free_count = 0;          // free-list length at the last sync
-    bool free_sorted = true;
+    int64_t live = 0;                 // paths in flight at the last sync
     unsigned long long last[6] = {0, 0, 0, 0, 0, 0};   // device statistics at the last sync
     std::vector<GenSeg> segs;         // the last refill's segments (kept until it ran)
 };
@@ -140,11 +138,6 @@ struct yn_ctx {
     // wavefront buffers
     int64_t cap_paths = 0;
     int64_t sample_cap = 0;           // samples buffer entries (frames x pixels per batch)
-    uint32_t* free_list = nullptr;    // slots of ended paths, for regeneration
-    uint32_t* free_alt = nullptr;     // the free list sorted by slot (double buffer)
-    void* sort_tmp = nullptr;         // radix-sort workspace
-    size_t sort_tmp_bytes = 0;
-    int slot_bits = 1;                // bits of a slot index
     PathSoA P{};
     RayQueue E[2]{}, Sh[2]{};
     DevCounters* ctr = nullptr;
@@ -240,28 +233,16 @@ int alloc_wavefront(yn_ctx* c, int64_t n)
     PathSoA& P = c->P;
     int r = 0;
     r |= A(&P.job); r |= A(&P.hit); r |= A(&P.occ);
-    {
-        auto B = [&](auto** q, size_t k) -> int {
-            int rr = dalloc(c, q, (size_t)n * k);
-            if (rr == 0) c->allocs.push_back(*q);
-            return rr;
-        };
-        r |= B(&P.g0, 2); r |= B(&P.g1, 4); r |= B(&P.g2, 4); r |= B(&P.g3, 4);
+    for (int k = 0; k < 2; ++k) {       // path records beside queues 0 and 1
+        PathRecs& R = P.R[k];
+        r |= A(&R.m0); r |= A(&R.m1); r |= A(&R.hp); r |= A(&R.hn); r |= A(&R.rd); r |= A(&R.tp);
+        r |= A(&R.direct); r |= A(&R.indirect); r |= A(&R.nd); r |= A(&R.bnc); r |= A(&R.wi); r |= A(&R.ls);
+        r |= A(&R.mis);
+        R.cap = n;
     }
     for (int k = 0; k < 2; ++k) {
         r |= A(&c->E[k].o_t); r |= A(&c->E[k].d_p);
         r |= A(&c->Sh[k].o_t); r |= A(&c->Sh[k].d_p);
-    }
-    r |= A(&c->free_list);
-    r |= A(&c->free_alt);
-    if (r == 0) {
-        c->slot_bits = 1;
-        while (((int64_t)1 << c->slot_bits) < n) ++c->slot_bits;
-        size_t tb = 0;
-        hipcub::DeviceRadixSort::SortKeys(nullptr, tb, c->free_list, c->free_alt, (int)n, 0, c->slot_bits, c->stream);
-        char* t = nullptr;
-        r |= dalloc(c, &t, std::max<size_t>(tb, 16));
-        if (t) { c->allocs.push_back(t); c->sort_tmp = t; c->sort_tmp_bytes = tb; }
     }
     {   // a batch may hold more work than paths in flight (regeneration), and
         // the continuous loop keeps a piece's samples until its last path
@@ -399,7 +380,6 @@ struct LoopCfg {
     int width = 2, max_depth = 0;
     int block = 256, grid = 1, shade_grid = 1;
     bool overlap = false, count = false;
-    uint32_t* free_list = nullptr;    // regeneration: ended paths hand their slots back here
     uint32_t* done_ctr = nullptr;     // continuous serving: ended paths per piece (entry - kMaxJobs)
 };
 
@@ -454,6 +434,7 @@ void make_iter(yn_ctx* c, const LoopCfg& L, int cur, IterArgs& A)
     ta.ovf_ref = c->ovf_ref;
     ta.ovf_t = c->ovf_t;
     ta.ovf_depth = c->ovf_depth;
+    ta.by_queue = L.bdpt ? 0 : 1;
     // extension rays
     ta.q = c->E[cur];
     ta.hit_out = c->P.hit;
@@ -494,7 +475,10 @@ void make_iter(yn_ctx* c, const LoopCfg& L, int cur, IterArgs& A)
         ShadeArgs& sa = A.sa;
         sa.scenes = c->d_scenes;
         sa.jobs = c->d_jobs;
-        sa.P = c->P;
+        sa.hit = c->P.hit;
+        sa.occ = c->P.occ;
+        sa.in = c->P.R[cur];
+        sa.out = c->P.R[nxt];
         sa.ext_in = c->E[cur];
         sa.sh_in = c->Sh[cur];
         sa.ext_out = c->E[nxt];
@@ -502,8 +486,6 @@ void make_iter(yn_ctx* c, const LoopCfg& L, int cur, IterArgs& A)
         sa.zero = &c->ctr->work[0][0][0];
         sa.n_zero = 2 * kMaxXcd;
         sa.ctr = c->ctr;
-        sa.free_list = L.free_list;
-        sa.free_count = q(10);
         sa.done_ctr = L.done_ctr;
         sa.done_base = kMaxJobs;      // continuous serving: pieces' entries start here
     }
@@ -591,6 +573,7 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     ga.scenes = scenes;
     ga.jobs = c->d_jobs;
     ga.P = c->P;
+    ga.rec = c->P.R[0];
     ga.ext_out = c->E[0];
     ga.segs = c->d_segs;
     ga.n_segs = (int32_t)segs.size();
@@ -608,7 +591,6 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     for (const Region& r : regs) scene_ids.push_back(c->jobs[r.job].scene);
     LoopCfg L;
     if (int r = loop_cfg(c, bdpt, scene_ids, stat_job.counting != 0, L)) return r;
-    L.free_list = regen ? c->free_list : nullptr;
     std::vector<Span> spans;
     IterArgs iters[2];
     make_iter(c, L, 0, iters[0]);
@@ -661,29 +643,19 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         if (it > 200000) return fail(c, YN_ERR_HIP, "wavefront loop did not drain");
         bool empty = bdpt ? c->h_ctr->q[8 + nxt][0] == 0 : (c->h_ctr->q[nxt][0] == 0 && c->h_ctr->q[2 + nxt][0] == 0);
         if (regen && issued < total) {
-            // refill the slots freed so far once there are enough of them (or
-            // the wavefront has drained), appending to the queue the next
-            // iteration reads
-            const uint32_t f = c->h_ctr->q[10][0];
-            const int64_t need = total - issued;
-            if (f > 0 && ((int64_t)f >= std::min<int64_t>(need, std::max<int64_t>(C / 8, 1)) || empty)) {
-                // slots come back in the order paths ended; sorted, the new
-                // paths take ascending slots and their state stays coalesced
-                size_t tb = c->sort_tmp_bytes;
-                HIPCK(c, hipcub::DeviceRadixSort::SortKeys(c->sort_tmp, tb, c->free_list, c->free_alt, (int)f, 0,
-                                                           c->slot_bits, s));
-                std::swap(c->free_list, c->free_alt);
-                L.free_list = c->free_list;  // the shade appends to the other buffer now
-                make_iter(c, L, 0, iters[0]);
-                make_iter(c, L, 1, iters[1]);
+            // new paths behind the rays of the queue the next iteration
+            // traces, once there is room for enough of them (or the
+            // wavefront has drained): a path's record sits beside its ray,
+            // so the room is the capacity less the rays queued
+            const int64_t queued = (int64_t)c->h_ctr->q[nxt][0] + c->h_ctr->q[2 + nxt][0];
+            const int64_t room = C - queued, need = total - issued;
+            if (room > 0 && (room >= std::min<int64_t>(need, std::max<int64_t>(C / 8, 1)) || empty)) {
                 GenArgs gr = ga;
                 gr.item0 = (int32_t)issued;
-                gr.n_paths = (int32_t)std::min<int64_t>((int64_t)f, need);
-                gr.free_list = c->free_list;
-                gr.free_top = f;
-                gr.free_count = q(10);
+                gr.n_paths = (int32_t)std::min<int64_t>(room, need);
                 gr.queue_base = q(11);
                 gr.ext_out = c->E[nxt];
+                gr.rec = c->P.R[nxt];
                 launch_generate(gr, cap, s);
                 issued += gr.n_paths;
                 continue;
@@ -975,8 +947,7 @@ int stream_start(yn_ctx* c)
     for (Piece& P : S.pieces) P = Piece();
     S.sfree.clear();
     S.sfree[0] = c->sample_cap;
-    S.free_count = (uint32_t)S.cap;
-    S.free_sorted = true;
+    S.live = 0;
     for (auto& v : S.last) v = 0;
     for (int k = 0; k < 2; ++k) {
         c->E[k].count = &c->ctr->q[k][0];
@@ -984,7 +955,6 @@ int stream_start(yn_ctx* c)
     }
     c->ev_used = 0;
     HIPCK(c, hipMemsetAsync(c->ctr, 0, sizeof(DevCounters), s));
-    launch_init_free(c->free_list, (int)S.cap, &c->ctr->q[10][0], s);
     HIPCK(c, hipGetLastError());
     return 0;
 }
@@ -1040,31 +1010,20 @@ int stream_plan(yn_ctx* c)
     return 0;
 }
 
-// Turns waiting work items into paths in the slots of ended paths: once at
-// least half the capacity is free (or all the waiting work fits, or the
-// wavefront is empty).  Path state is indexed by slot, so new paths whose
-// slots lie far apart cost the shade pass scattered accesses: refilling at
-// half the capacity (27 refills for the 64-job mix) instead of an eighth (45)
-// cut its shade time by 7%.  The new paths' rays are appended to the queue
-// the next iteration traces.
+// Turns waiting work items into paths: their rays are appended behind the
+// rays the next iteration traces, their records beside them.  Once at least
+// an eighth of the capacity is free (or all the waiting work fits, or the
+// wavefront is empty).
 int stream_refill(yn_ctx* c)
 {
     StreamState& S = c->strm;
     const int64_t pending = stream_pending(c);
-    const int64_t f = S.free_count;
-    if (pending == 0 || f == 0) return 0;
-    const bool empty = f >= S.cap;
-    if (!empty && f < std::min<int64_t>(pending, std::max<int64_t>(S.cap / 2, 1))) return 0;
+    const int64_t f = S.cap - S.live;
+    if (pending == 0 || f <= 0) return 0;
+    const bool empty = S.live == 0;
+    if (!empty && f < std::min<int64_t>(pending, std::max<int64_t>(S.cap / 8, 1))) return 0;
     const int64_t n = std::min<int64_t>(f, pending);
     hipStream_t s = c->stream;
-    if (!S.free_sorted && f > 1) {
-        // slots come back in the order paths ended; sorted, new paths take
-        // ascending slots and their state stays grouped
-        size_t tb = c->sort_tmp_bytes;
-        HIPCK(c, hipcub::DeviceRadixSort::SortKeys(c->sort_tmp, tb, c->free_list, c->free_alt, (int)f, 0,
-                                                   c->slot_bits, s));
-        std::swap(c->free_list, c->free_alt);
-    }
     // segments: items of the waiting pieces in order, frame by frame
     S.segs.clear();
     int64_t left = n, base = 0;
@@ -1095,20 +1054,17 @@ int stream_refill(yn_ctx* c)
     g.scenes = c->d_scenes;
     g.jobs = c->d_jobs;
     g.P = c->P;
+    g.rec = c->P.R[S.it & 1];
     g.ext_out = c->E[S.it & 1];
     g.segs = c->d_segs;
     g.n_segs = (int32_t)S.segs.size();
     g.n_paths = (int32_t)n;
     g.item0 = 0;
-    g.free_list = c->free_list;
-    g.free_top = (uint32_t)f;
-    g.free_count = &c->ctr->q[10][0];
     g.queue_base = &c->ctr->q[11][0];
     launch_generate(g, DevCapture{}, s);
     HIPCK(c, hipGetLastError());
     c->serve_rec.st.refills++;
-    S.free_count = (uint32_t)(f - n);
-    S.free_sorted = true;         // what is left is the sorted list's lower part
+    S.live += n;
     return 0;
 }
 
@@ -1161,7 +1117,7 @@ int stream_group(yn_ctx* c, bool plan, bool* ran)
     if (plan)
         if (int r = stream_plan(c)) return r;
     if (int r = stream_refill(c)) return r;
-    if ((int64_t)S.free_count >= S.cap) {
+    if (S.live == 0) {
         // nothing in flight: every issued path has ended
         if (int r = stream_fold(c)) return r;
         if (S.issue.empty() && !stream_has_pieces(c)) S.active = false;
@@ -1172,7 +1128,6 @@ int stream_group(yn_ctx* c, bool plan, bool* ran)
         if (P.live) scene_ids.push_back(P.scene);
     LoopCfg L;
     if (int r = loop_cfg(c, false, scene_ids, false, L)) return r;
-    L.free_list = c->free_list;
     L.done_ctr = c->done_ctr;
     IterArgs iters[2];
     make_iter(c, L, 0, iters[0]);
@@ -1180,7 +1135,7 @@ int stream_group(yn_ctx* c, bool plan, bool* ran)
     std::vector<Span> spans;
     c->ev_used = 0;
     c->serve_rec.st.groups++;
-    c->serve_rec.st.paths_in_flight += (uint64_t)(S.cap - (int64_t)S.free_count);
+    c->serve_rec.st.paths_in_flight += (uint64_t)S.live;
     c->serve_rec.st.ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
     for (int g = 0; g < 4; ++g, ++S.it)
         if (int r = enqueue_iter(c, L, iters[S.it & 1], DevCapture{}, &spans)) return r;
@@ -1205,8 +1160,8 @@ int stream_group(yn_ctx* c, bool plan, bool* ran)
     st.iterations += 4;
     st.trace_launches_ext += 4;
     st.trace_launches_shadow += 4;
-    S.free_count = c->h_ctr->q[10][0];
-    S.free_sorted = false;
+    // every path in flight has one ray in the queues the next iteration reads
+    S.live = (int64_t)c->h_ctr->q[S.it & 1][0] + c->h_ctr->q[2 + (S.it & 1)][0];
     return stream_fold(c);
 }
 

@@ -459,7 +459,9 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
             float4 ot = a.q.o_t[i];
             float4 dp = a.q.d_p[i];
             int pidx = __float_as_int(dp.w);
-            int sc = a.single_scene >= 0 ? a.single_scene : a.jobs[a.path_job[pidx >> a.id_shift]].scene;
+            int sc = a.single_scene >= 0 ? a.single_scene
+                                         : a.jobs[a.by_queue ? pidx : a.path_job[pidx >> a.id_shift]].scene;
+            if (a.by_queue) pidx = (int)i;      // the result goes beside the ray
             const DevScene& S = a.scenes[sc];
             RayState r;
             r.ox = ot.x; r.oy = ot.y; r.oz = ot.z;
@@ -591,7 +593,9 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
                 float4 ot = a.q.o_t[i];
                 float4 dp = a.q.d_p[i];
                 pidx = __float_as_int(dp.w);
-                int sc = a.single_scene >= 0 ? a.single_scene : a.jobs[a.path_job[pidx >> a.id_shift]].scene;
+                int sc = a.single_scene >= 0 ? a.single_scene
+                                             : a.jobs[a.by_queue ? pidx : a.path_job[pidx >> a.id_shift]].scene;
+                if (a.by_queue) pidx = (int)i;      // the result goes beside the ray
                 const DevScene& S = a.scenes[sc];
                 r.ox = ot.x; r.oy = ot.y; r.oz = ot.z;
                 r.dx = dp.x; r.dy = dp.y; r.dz = dp.z;

@@ -41,18 +41,43 @@ struct DevJob {
     float4* samples;            // [slots][W*H] one sample colour per frame slot
 };
 
-// A queue of rays: rays[i] = {o.xyz, tmax}, {d.xyz, path index bits}.
+// A queue of rays: rays[i] = {o.xyz, tmax}, {d.xyz, w}.  w: the path index
+// (bidirectional paths, caller rays), or the job-table entry of the path
+// whose state travels with the ray (unidirectional paths).
 struct RayQueue {
     float4* o_t;
     float4* d_p;
     uint32_t* count;            // number of rays in the queue
 };
 
-// Path states, structure of arrays, one entry per path in flight.
+// Unidirectional path state, one record per ray in a pair of queues: the
+// record of extension-queue entry i is record i, that of shadow-queue entry
+// j is record cap - 1 - j.  A shade step reads the record beside the ray
+// whose result came back and writes it, updated, beside the ray it makes,
+// so records move with their rays in queue order and every access is
+// coalesced.
+struct PathRecs {
+    int4* m0;                   // {seed, state | flags, bounce, material of the current vertex}
+    int4* m1;                   // {pixel, job-table entry, frame slot, NEE lobe probability bits}
+    float4* hp;                 // vertex position
+    float4* hn;                 // shading normal
+    float4* rd;                 // incoming direction (the camera ray's for a new path)
+    float4* tp;                 // throughput
+    float4* direct;             // direct light of the first vertex
+    float4* indirect;           // everything else
+    float4* nd;                 // pending bounce direction
+    float4* bnc;                // pending bounce: pdf, lobe probability, ior
+    float4* wi;                 // NEE: light direction, w = light pdf
+    float4* ls;                 // NEE: MIS-weighted light sample
+    float4* mis;                // NEE: MIS BRDF direction, w = BRDF pdf
+    int64_t cap;                // records (paths in flight at most)
+};
+
+// Path states, structure of arrays.
 struct PathSoA {
-    int32_t* job;               // job slot (read by the trace kernels' scene lookup)
-    float4* hit;                // closest-hit result: t, tri bits, u, v
-    int32_t* occ;               // any-hit result
+    int32_t* job;               // bidirectional: job slot per path (the trace kernels' scene lookup)
+    float4* hit;                // closest-hit result: t, tri bits, u, v (per path / queue entry)
+    int32_t* occ;               // any-hit result (per shadow-queue entry)
     // bidirectional mode only (allocated with its state, one entry per path):
     uint32_t* seed;
     int32_t* pix;               // pixel index y*W + x
@@ -66,13 +91,7 @@ struct PathSoA {
     float4* ls;                 // NEE: light sample (MIS-weighted)
     float4* mis;                // NEE: MIS brdf dir xyz, w = brdf pdf
     float* nee_prob;            // NEE: lobe probability of the light sample
-    // Unidirectional shading state, grouped by use: a shade step reads and
-    // writes whole 32/64-byte records instead of one cache line per field,
-    // which matters because a queue's paths are in no particular slot order.
-    int4* g0;                   // [2 per path] {seed, state, bounce, material of the vertex}, {pix, job, slot, nee_prob bits}
-    float4* g1;                 // [4 per path] hp, hn, rd, tp
-    float4* g2;                 // [4 per path] direct, indirect, nd, bnc
-    float4* g3;                 // [4 per path] wi (w: light pdf), ls, mis, -
+    PathRecs R[2];              // unidirectional: records beside queues 0 and 1
 };
 
 // Bidirectional paths (bdpt.cl): both subpaths' vertices plus the

@@ -28,6 +28,8 @@ struct TraceArgs {
     uint16_t* ovf_t;            //   per grid thread, strided by the grid size
     int32_t ovf_depth;
     int32_t id_shift;           // ray id >> id_shift = path index (result slot = ray id)
+    int32_t by_queue;           // 1: a ray's w is its job-table entry and its result goes
+                                //    to its queue position (unidirectional paths)
 };
 
 // Segment of primary rays: a rectangle of one job's image at one frame.
@@ -48,14 +50,14 @@ constexpr int kShadeBlockPT = 512;
 struct ShadeArgs {
     const DevScene* scenes;
     const DevJob* jobs;
-    PathSoA P;
+    const float4* hit;          // closest-hit results, per extension-queue entry
+    const int32_t* occ;         // any-hit results, per shadow-queue entry
+    PathRecs in, out;           // records beside the input and the output queues
     RayQueue ext_in, sh_in;     // results to consume
     RayQueue ext_out, sh_out;   // rays to produce
     uint32_t* zero;             // work counters to clear for the next traces
     int32_t n_zero;
     DevCounters* ctr;
-    uint32_t* free_list;        // slots of ended paths (regeneration), or null
-    uint32_t* free_count;
     uint32_t* done_ctr;         // continuous serving: paths ended per job-table entry from
     int32_t done_base;          //   done_base on (done_ctr[entry - done_base]), or null
 };
@@ -80,7 +82,8 @@ struct BdptShadeArgs {
 struct GenArgs {
     const DevScene* scenes;
     const DevJob* jobs;
-    PathSoA P;
+    PathSoA P;                  // bidirectional mode
+    PathRecs rec;               // unidirectional mode: the records beside ext_out
     RayQueue ext_out;
     const GenSeg* segs;
     int32_t n_segs;
@@ -88,15 +91,12 @@ struct GenArgs {
     BdptSoA B;                  // bidirectional mode only
     uint32_t* list_out;         // bidirectional mode: every path starts listed
     uint32_t* list_count;
-    // regeneration: work items [item0, item0 + n_paths) take the slots
-    // free_list[free_top - n_paths + i] (the list's top n_paths entries, in
-    // list order) and append their rays to ext_out; without a free list,
-    // item i takes slot i and ray position i
+    // work items [item0, item0 + n_paths); with queue_base, item i is
+    // appended behind the rays already in ext_out (at *queue_base + i, which
+    // k_queue_reserve sets just before), else it takes ray position i and
+    // the launch sets the queue's count
     int32_t item0;
-    const uint32_t* free_list;
-    uint32_t free_top;
-    uint32_t* free_count;
-    uint32_t* queue_base;       // with a free list: where the new rays start in ext_out
+    uint32_t* queue_base;
 };
 
 struct ResolveArgs {
@@ -231,7 +231,6 @@ void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, int wa
 void launch_generate_bdpt(const GenArgs& a, const DevCapture& cap, hipStream_t stream);
 void launch_shade_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int grid, hipStream_t stream);
 void launch_resolve(const ResolveArgs& a, hipStream_t stream);
-void launch_init_free(uint32_t* list, int n, uint32_t* count, hipStream_t stream);
 void launch_fmath(int op, const float* x, const float* y, int n, float* out, hipStream_t stream);
 void launch_tonemap(const float4* in, float4* out, int n, int op, hipStream_t stream);
 void launch_prepare(const DevScene* scene, const float* rays, int n, RayQueue q, int32_t* light_out,
