@@ -85,24 +85,25 @@ __device__ __forceinline__ Out make_ray(int kind, f4 o, f4 d, float tmax, int jo
 // only m0, m1 and the camera ray's direction.
 __device__ __forceinline__ void load_rec(const PathRecs& I, uint32_t ri, Rec& R)
 {
-    R.m0 = I.m0[ri];
-    R.m1 = I.m1[ri];
-    R.rd = ld(I.rd[ri]);
+    using F = PathRecs;
+    R.m0 = *reinterpret_cast<const int4*>(I.f(F::M0, ri));
+    R.m1 = *reinterpret_cast<const int4*>(I.f(F::M1, ri));
+    R.rd = ld(*I.f(F::RD, ri));
     const int code = R.m0.y & ST_MASK;
     if (code == ST_PRIMARY) return;
-    R.hp = ld(I.hp[ri]);
-    R.hn = ld(I.hn[ri]);
-    R.tp = ld(I.tp[ri]);
-    R.direct = ld(I.direct[ri]);
-    R.indirect = ld(I.indirect[ri]);
+    R.hp = ld(*I.f(F::HP, ri));
+    R.hn = ld(*I.f(F::HN, ri));
+    R.tp = ld(*I.f(F::TP, ri));
+    R.direct = ld(*I.f(F::DIRECT, ri));
+    R.indirect = ld(*I.f(F::INDIRECT, ri));
     if (code == ST_BOUNCE) {
-        R.nd = ld(I.nd[ri]);
-        R.bnc = ld(I.bnc[ri]);
+        R.nd = ld(*I.f(F::ND, ri));
+        R.bnc = ld(*I.f(F::BNC, ri));
     } else {
-        R.wi = ld(I.wi[ri]);
+        R.wi = ld(*I.f(F::WI, ri));
         if (code == ST_NEE_MIS) {
-            R.ls = ld(I.ls[ri]);
-            R.mis = ld(I.mis[ri]);
+            R.ls = ld(*I.f(F::LS, ri));
+            R.mis = ld(*I.f(F::MIS, ri));
         }
     }
 }
@@ -110,23 +111,24 @@ __device__ __forceinline__ void load_rec(const PathRecs& I, uint32_t ri, Rec& R)
 // The record beside a ray just made: what the next step's state uses.
 __device__ __forceinline__ void store_rec(const PathRecs& O, uint32_t ri, const Rec& R)
 {
-    O.m0[ri] = R.m0;
-    O.m1[ri] = R.m1;
-    O.hp[ri] = st4(R.hp);
-    O.hn[ri] = st4(R.hn);
-    O.rd[ri] = st4(R.rd);
-    O.tp[ri] = st4(R.tp);
-    O.direct[ri] = st4(R.direct);
-    O.indirect[ri] = st4(R.indirect);
+    using F = PathRecs;
+    *reinterpret_cast<int4*>(O.f(F::M0, ri)) = R.m0;
+    *reinterpret_cast<int4*>(O.f(F::M1, ri)) = R.m1;
+    *O.f(F::HP, ri) = st4(R.hp);
+    *O.f(F::HN, ri) = st4(R.hn);
+    *O.f(F::RD, ri) = st4(R.rd);
+    *O.f(F::TP, ri) = st4(R.tp);
+    *O.f(F::DIRECT, ri) = st4(R.direct);
+    *O.f(F::INDIRECT, ri) = st4(R.indirect);
     const int code = R.m0.y & ST_MASK;
     if (code == ST_BOUNCE) {
-        O.nd[ri] = st4(R.nd);
-        O.bnc[ri] = st4(R.bnc);
+        *O.f(F::ND, ri) = st4(R.nd);
+        *O.f(F::BNC, ri) = st4(R.bnc);
     } else {
-        O.wi[ri] = st4(R.wi);
+        *O.f(F::WI, ri) = st4(R.wi);
         if (code == ST_NEE_MIS) {
-            O.ls[ri] = st4(R.ls);
-            O.mis[ri] = st4(R.mis);
+            *O.f(F::LS, ri) = st4(R.ls);
+            *O.f(F::MIS, ri) = st4(R.mis);
         }
     }
 }
@@ -506,9 +508,9 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a, DevCapture cap)
     float tq, tmax = INFINITY;
     int st = ST_PRIMARY;
     if (quad_hit(S.light, o, d, INFINITY, tq)) { tmax = tq; st |= F_LCAND; }
-    a.rec.m0[qpos] = make_int4((int)seed, st, 0, -1);
-    a.rec.m1[qpos] = make_int4(y * J.W + x, g.job, g.slot, 0);
-    a.rec.rd[qpos] = st4(d);
+    *reinterpret_cast<int4*>(a.rec.f(PathRecs::M0, qpos)) = make_int4((int)seed, st, 0, -1);
+    *reinterpret_cast<int4*>(a.rec.f(PathRecs::M1, qpos)) = make_int4(y * J.W + x, g.job, g.slot, 0);
+    *a.rec.f(PathRecs::RD, qpos) = st4(d);
     a.ext_out.o_t[qpos] = make_float4(o.x, o.y, o.z, tmax);
     a.ext_out.d_p[qpos] = make_float4(d.x, d.y, d.z, __int_as_float(g.job));
     if (!a.queue_base && i == 0) *a.ext_out.count = (uint32_t)a.n_paths;

@@ -235,9 +235,9 @@ int alloc_wavefront(yn_ctx* c, int64_t n)
     r |= A(&P.job); r |= A(&P.hit); r |= A(&P.occ);
     for (int k = 0; k < 2; ++k) {       // path records beside queues 0 and 1
         PathRecs& R = P.R[k];
-        r |= A(&R.m0); r |= A(&R.m1); r |= A(&R.hp); r |= A(&R.hn); r |= A(&R.rd); r |= A(&R.tp);
-        r |= A(&R.direct); r |= A(&R.indirect); r |= A(&R.nd); r |= A(&R.bnc); r |= A(&R.wi); r |= A(&R.ls);
-        r |= A(&R.mis);
+        int rr = dalloc(c, &R.base, (size_t)n * PathRecs::FIELDS);
+        if (rr == 0) c->allocs.push_back(R.base);
+        r |= rr;
         R.cap = n;
     }
     for (int k = 0; k < 2; ++k) {

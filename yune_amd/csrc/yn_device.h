@@ -57,20 +57,29 @@ struct RayQueue {
 // so records move with their rays in queue order and every access is
 // coalesced.
 struct PathRecs {
-    int4* m0;                   // {seed, state | flags, bounce, material of the current vertex}
-    int4* m1;                   // {pixel, job-table entry, frame slot, NEE lobe probability bits}
-    float4* hp;                 // vertex position
-    float4* hn;                 // shading normal
-    float4* rd;                 // incoming direction (the camera ray's for a new path)
-    float4* tp;                 // throughput
-    float4* direct;             // direct light of the first vertex
-    float4* indirect;           // everything else
-    float4* nd;                 // pending bounce direction
-    float4* bnc;                // pending bounce: pdf, lobe probability, ior
-    float4* wi;                 // NEE: light direction, w = light pdf
-    float4* ls;                 // NEE: MIS-weighted light sample
-    float4* mis;                // NEE: MIS BRDF direction, w = BRDF pdf
+    // Thirteen 16-byte fields per record, each field an array of `cap`
+    // entries in one allocation: field f of record r at base[f * cap + r].
+    // One base pointer (not thirteen) keeps the kernels' argument registers
+    // few.
+    float4* base;
     int64_t cap;                // records (paths in flight at most)
+    enum Field {
+        M0,                     // int4 {seed, state | flags, bounce, material of the current vertex}
+        M1,                     // int4 {pixel, job-table entry, frame slot, NEE lobe probability bits}
+        HP,                     // vertex position
+        HN,                     // shading normal
+        RD,                     // incoming direction (the camera ray's for a new path)
+        TP,                     // throughput
+        DIRECT,                 // direct light of the first vertex
+        INDIRECT,               // everything else
+        ND,                     // pending bounce direction
+        BNC,                    // pending bounce: pdf, lobe probability, ior
+        WI,                     // NEE: light direction, w = light pdf
+        LS,                     // NEE: MIS-weighted light sample
+        MIS,                    // NEE: MIS BRDF direction, w = BRDF pdf
+        FIELDS
+    };
+    __host__ __device__ float4* f(int field, int64_t r) const { return base + (size_t)field * (size_t)cap + (size_t)r; }
 };
 
 // Path states, structure of arrays.
