@@ -55,10 +55,22 @@ __device__ __forceinline__ bool box_hit(float lox, float loy, float loz, float h
                                         float ox, float oy, float oz, float ix, float iy, float iz,
                                         float& enter)
 {
-    float tmin = -INFINITY, tmax = INFINITY;
-    slab<EXACT>(lox, hix, ox, ix, tmin, tmax);
-    slab<EXACT>(loy, hiy, oy, iy, tmin, tmax);
-    slab<EXACT>(loz, hiz, oz, iz, tmin, tmax);
+    float tmin, tmax;
+    if (EXACT) {
+        tmin = -INFINITY;
+        tmax = INFINITY;
+        slab<EXACT>(lox, hix, ox, ix, tmin, tmax);
+        slab<EXACT>(loy, hiy, oy, iy, tmin, tmax);
+        slab<EXACT>(loz, hiz, oz, iz, tmin, tmax);
+    } else {
+        // no NaN can occur: the same maxima and minima without the
+        // reference's -inf / +inf starting values (one min per box fewer)
+        const float ax = (lox - ox) * ix, bx = (hix - ox) * ix;
+        const float ay = (loy - oy) * iy, by = (hiy - oy) * iy;
+        const float az = (loz - oz) * iz, bz = (hiz - oz) * iz;
+        tmin = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+        tmax = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    }
     enter = fmaxf(tmin, 0.0f);
     return tmax > enter;
 }
