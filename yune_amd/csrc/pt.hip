@@ -46,6 +46,7 @@ enum : int {
     F_NGLOSSY = 1 << 10,    // NEE lobe: glossy
     F_BGLOSSY = 1 << 11,    // bounce lobe: glossy
     F_CAPSH = 1 << 12,      // first shadow ray already captured
+    SCENE_SHIFT = 16,       // bits 16..23: the path's scene (its job's), read without the job
 };
 
 constexpr float kPinkR = 0.988f, kPinkG = 0.0588f, kPinkB = 0.7529f;   // udpt.cl:110
@@ -81,30 +82,28 @@ __device__ __forceinline__ Out make_ray(int kind, f4 o, f4 d, float tmax, int jo
     return r;
 }
 
-// The record a step reads: what its state uses.  A new path's record holds
-// only m0, m1 and the camera ray's direction.
+// The record a step reads.  The fields most states use are loaded without
+// waiting for the state (a new path's record holds only m0, m1 and the
+// camera ray's direction; the rest is not used then); only the MIS ray's
+// fields, used by one state, wait for it.  The shade pass is bound by the
+// latency of such chains of loads.
 __device__ __forceinline__ void load_rec(const PathRecs& I, uint32_t ri, Rec& R)
 {
     using F = PathRecs;
     R.m0 = *reinterpret_cast<const int4*>(I.f(F::M0, ri));
     R.m1 = *reinterpret_cast<const int4*>(I.f(F::M1, ri));
     R.rd = ld(*I.f(F::RD, ri));
-    const int code = R.m0.y & ST_MASK;
-    if (code == ST_PRIMARY) return;
     R.hp = ld(*I.f(F::HP, ri));
     R.hn = ld(*I.f(F::HN, ri));
     R.tp = ld(*I.f(F::TP, ri));
     R.direct = ld(*I.f(F::DIRECT, ri));
     R.indirect = ld(*I.f(F::INDIRECT, ri));
-    if (code == ST_BOUNCE) {
-        R.nd = ld(*I.f(F::ND, ri));
-        R.bnc = ld(*I.f(F::BNC, ri));
-    } else {
-        R.wi = ld(*I.f(F::WI, ri));
-        if (code == ST_NEE_MIS) {
-            R.ls = ld(*I.f(F::LS, ri));
-            R.mis = ld(*I.f(F::MIS, ri));
-        }
+    R.nd = ld(*I.f(F::ND, ri));
+    R.bnc = ld(*I.f(F::BNC, ri));
+    R.wi = ld(*I.f(F::WI, ri));
+    if ((R.m0.y & ST_MASK) == ST_NEE_MIS) {
+        R.ls = ld(*I.f(F::LS, ri));
+        R.mis = ld(*I.f(F::MIS, ri));
     }
 }
 
@@ -140,6 +139,7 @@ __device__ __forceinline__ void store_rec(const PathRecs& O, uint32_t ri, const 
 // camera's eye (createRay, udpt.cl:236) along the stored direction rd; a
 // bounce ray at hp + nd * EPSILON along the pending bounce direction nd
 // (udpt.cl:766, 906, 945).
+template <bool CAP>
 __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint32_t qi, Rec& R)
 {
     load_rec(a.in, ext ? qi : (uint32_t)(a.in.cap - 1 - (int64_t)qi), R);
@@ -147,7 +147,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint
     const int jid = m1.y;
     const DevJob& J = a.jobs[jid];
     const yn_render_params& RP = J.p;
-    const DevScene& S = a.scenes[J.scene];
+    const DevScene& S = a.scenes[(m0.y >> SCENE_SHIFT) & 0xff];     // loaded beside J, not after it
     const yn_light_quad& L = S.light;
     const bool on = RP.oren_nayar != 0;
     const bool fix = RP.fix_mis != 0;
@@ -157,7 +157,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint
     uint32_t seed = (uint32_t)m0.x;
     const int pix = m1.x;
     const int slot = m1.z;
-    const bool cap_on = cap.prim_tri != nullptr && slot == 0;
+    const bool cap_on = CAP && cap.prim_tri != nullptr && slot == 0;
 
     int vmat = m0.w;                    // material of the path's current vertex
     f4& hp = R.hp;
@@ -404,14 +404,17 @@ steps_done:
 }  // namespace
 
 // WAVES: the occupancy the register allocation is held to, per SIMD.
-// Blocks of kShadeBlockPT threads walk the queues together, one atomic per
+// Blocks of shade_block_pt(WAVES) threads walk the queues together, one atomic per
 // block and output queue.  A path's record is read beside the ray whose
 // result it consumes and written, once the block knows where the path's
 // next ray goes, beside that ray.
-template <int WAVES>
-__global__ __launch_bounds__(kShadeBlockPT, WAVES) void k_shade_pt(ShadeArgs a, DevCapture cap)
+// CAP: the variant that records the first frame's primary hits and shadow
+// rays for the parity tests (without it, the kernel holds fewer values).
+template <int WAVES, bool CAP>
+__global__ __launch_bounds__(shade_block_pt(WAVES), WAVES) void k_shade_pt(ShadeArgs a, DevCapture cap)
 {
-    __shared__ BlockReserveLds<2, kShadeBlockPT> rsv;
+    constexpr int BL = shade_block_pt(WAVES);
+    __shared__ BlockReserveLds<2, BL> rsv;
     // continuous serving: ended paths per job-table entry, gathered per block
     // in a small table and added to the device counters once at the end
     // (device-scope atomics on one cache line, one per wave, serialise)
@@ -436,18 +439,18 @@ __global__ __launch_bounds__(kShadeBlockPT, WAVES) void k_shade_pt(ShadeArgs a, 
     uint32_t* const ctr[2] = {a.ext_out.count, a.sh_out.count};
     int par = 0;
     // the block walks the queues together: its appends are one atomic per queue
-    constexpr uint32_t B = kShadeBlockPT;
+    constexpr uint32_t B = BL;
     for (uint32_t base = blockIdx.x * B; base < total; base += gridDim.x * B, par ^= 1) {
         uint32_t i = base + threadIdx.x;
         Out o;
         Rec R;
         if (i < total) {
             const bool ext = i < n_ext;
-            o = advance(a, cap, ext, ext ? i : i - n_ext, R);
+            o = advance<CAP>(a, cap, ext, ext ? i : i - n_ext, R);
         }
         const uint32_t n[2] = {o.kind == 1 ? 1u : 0u, o.kind == 2 ? 1u : 0u};
         uint32_t at[2];
-        block_reserve<2, kShadeBlockPT>(rsv, par, ctr, n, at);
+        block_reserve<2, BL>(rsv, par, ctr, n, at);
         if (o.kind == 1) {
             a.ext_out.o_t[at[0]] = o.o_t;
             a.ext_out.d_p[at[0]] = o.d_p;
@@ -506,7 +509,7 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a, DevCapture cap)
     f4 o, d;
     create_ray((float)x + r1, (float)y + r2, J.W, J.H, J.cam, o, d);
     float tq, tmax = INFINITY;
-    int st = ST_PRIMARY;
+    int st = ST_PRIMARY | (J.scene << SCENE_SHIFT);
     if (quad_hit(S.light, o, d, INFINITY, tq)) { tmax = tq; st |= F_LCAND; }
     *reinterpret_cast<int4*>(a.rec.f(PathRecs::M0, qpos)) = make_int4((int)seed, st, 0, -1);
     *reinterpret_cast<int4*>(a.rec.f(PathRecs::M1, qpos)) = make_int4(y * J.W + x, g.job, g.slot, 0);
@@ -630,9 +633,13 @@ void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t s)
 }
 void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, int waves, hipStream_t s)
 {
-    if (waves == 4) hipLaunchKernelGGL(k_shade_pt<4>, dim3(grid), dim3(kShadeBlockPT), 0, s, a, cap);
-    else if (waves == 3) hipLaunchKernelGGL(k_shade_pt<3>, dim3(grid), dim3(kShadeBlockPT), 0, s, a, cap);
-    else hipLaunchKernelGGL(k_shade_pt<2>, dim3(grid), dim3(kShadeBlockPT), 0, s, a, cap);
+    if (cap.prim_tri) {
+        hipLaunchKernelGGL((k_shade_pt<4, true>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
+        return;
+    }
+    if (waves == 4) hipLaunchKernelGGL((k_shade_pt<4, false>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
+    else if (waves == 3) hipLaunchKernelGGL((k_shade_pt<3, false>), dim3(grid), dim3(shade_block_pt(3)), 0, s, a, cap);
+    else hipLaunchKernelGGL((k_shade_pt<2, false>), dim3(grid), dim3(shade_block_pt(2)), 0, s, a, cap);
 }
 void launch_resolve(const ResolveArgs& a, hipStream_t s)
 {

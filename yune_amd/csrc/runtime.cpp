@@ -407,7 +407,7 @@ int loop_cfg(yn_ctx* c, bool bdpt, const std::vector<int>& scene_ids, bool count
                                    : c->n_cu * std::max(1, trace_resident_waves(c->cfg.lds_stack) * 256 / L.block);
     // persistent shade grids: the blocks that fit at once (4 SIMDs per CU;
     // the bidirectional shade runs 3 waves per SIMD in blocks of 256)
-    L.shade_grid = bdpt ? c->n_cu * 3 : std::max(1, c->n_cu * c->cfg.shade_waves * 256 / kShadeBlockPT);
+    L.shade_grid = bdpt ? c->n_cu * 3 : c->n_cu * 2;
     L.overlap = c->cfg.trace_overlap != 0;
     // overlapping traces get disjoint halves of the stack overflow area
     if (int r = ensure_overflow(c, L.max_depth - trace_lds_stack(c->cfg.lds_stack),

@@ -43,9 +43,11 @@ struct GenSeg {
     int32_t first;              // pixel of the rectangle its first item renders
 };
 
-// Threads per block of the path-tracing shade: its queue appends are one
-// atomic per block, and 512 (8 waves) beat 256 by 1-3% and 1024 by 1%.
-constexpr int kShadeBlockPT = 512;
+// Threads per block of the path-tracing shade compiled for `waves` waves
+// per SIMD: two blocks fill a CU (4 SIMDs).  Its queue appends are one
+// atomic per block; at 4 waves per SIMD, blocks of 512 beat 256 by 1-3%
+// and 1024 by 1%.
+constexpr int shade_block_pt(int waves) { return waves * 128; }
 
 struct ShadeArgs {
     const DevScene* scenes;
