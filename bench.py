@@ -225,14 +225,14 @@ def serve_run(Y, np, serve, rate, duration, n_jobs, plain_jobs, frame_jobs, serv
     served = len(jobs) / makespan
     samples = sum(j["spp"] * j["W"] * j["H"] for j in jobs)
 
-    def rate(idx, t):
+    def list_rate(idx, t):
         """jobs/s of a plain path over the whole list: its time on the sampled
         jobs scaled by the list's samples over theirs (all jobs: exact)"""
         part = sum(jobs[k]["spp"] * jobs[k]["W"] * jobs[k]["H"] for k in idx)
         return len(jobs) / (t * samples / part)
 
-    plain_job = rate(pj, t_job)
-    plain_frame = rate(fj, t_frame)
+    plain_job = list_rate(pj, t_job)
+    plain_frame = list_rate(fj, t_frame)
     return {
         "value": round(served, 3), "unit": "jobs/s",
         "workload": ("Poisson arrivals at %.1f jobs/s for %.1f s" % (rate, duration) if serve else

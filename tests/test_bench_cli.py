@@ -51,3 +51,69 @@ def test_roofline_rows_from_a_summary():
     assert abs(row["achieved"] - 800 * 16e6 / 5e-3 / 1e9) < 0.1
     assert abs(row["frac"] - row["achieved"] / 8000.0) < 1e-4
     assert row["algorithmic"]["bytes_per_unit"] == 2500.0 and row["valu_busy"] == 0.7
+
+
+def test_serve_bookkeeping_with_a_stand_in_library():
+    """serve_run's arrival loop, latency and plain-path bookkeeping, driven
+    by a stand-in for the library (no GPU): every job is done one clock tick
+    after submission."""
+    import types
+
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    import bench
+
+    class Job:
+        def __init__(self, ctx):
+            self.ctx = ctx
+
+        def status(self):
+            return {"done": True, "t_done_ms": self.ctx.t + 1.0}
+
+        def read(self):
+            return np.zeros((2, 2, 4), np.float32)
+
+        def release(self):
+            pass
+
+        def render(self, n):
+            pass
+
+    class Ctx:
+        def __init__(self, **kw):
+            self.t = 0.0
+
+        def __enter__(self):
+            return self
+
+        def __exit__(self, *a):
+            pass
+
+        def scene(self, s):
+            return 0
+
+        def job(self, *a, **k):
+            return Job(self)
+
+        def submit(self, *a):
+            return Job(self)
+
+        def clock_ms(self):
+            self.t += 5.0
+            return self.t
+
+        def serve(self, n):
+            return 0
+
+        def serve_stats(self):
+            return types.SimpleNamespace(rays_ext=10, rays_shadow=5, ms_trace_ext=1.0, ms_trace_shadow=1.0,
+                                         ms_shade=1.0, ms_host=0.1, groups=1, refills=1, plan_stalls=0,
+                                         paths_in_flight=3)
+
+    Y = types.SimpleNamespace(Context=Ctx, gen_icosphere_room=lambda *a: types.SimpleNamespace(tris=[0] * 10),
+                              params_reference=lambda **k: None, camera=lambda *a: None)
+    r = bench.serve_run(Y, np, True, 20.0, 1.0, 0, 5, 2)
+    assert r["jobs"] == 20 and r["plain"]["job_at_a_time_jobs"] == 5 and r["plain"]["frame_at_a_time_jobs"] == 2
+    assert r["latency_ms"]["p99"] >= r["latency_ms"]["p50"] and r["plain"]["images_identical_to_served"]
+    r = bench.serve_run(Y, np, False, 20.0, 1.0, 0, 0, 2)
+    assert r["jobs"] == 64 and r["plain"]["job_at_a_time_jobs"] == 64

@@ -107,6 +107,7 @@ __device__ __forceinline__ void set_ray(BOut& r, f4 o, f4 d, float tmax, int id)
     r.d_p = make_float4(d.x, d.y, d.z, __int_as_float(id));
 }
 
+template <bool CAP>
 __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int p)
 {
     const PathSoA& P = a.P;
@@ -123,7 +124,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
     uint32_t seed = P.seed[p];
     const int pix = P.pix[p];
     const int slot = P.slot[p];
-    const bool cap_on = cap.prim_tri != nullptr && slot == 0;
+    const bool cap_on = CAP && cap.prim_tri != nullptr && slot == 0;
     int k = P.bounce[p];
     int lp_len = B.lp_len[p], ep_len = B.ep_len[p];
     float w_eye = B.w_eye[p];
@@ -422,6 +423,9 @@ emitted:
 // Compiled for 3 waves per SIMD (168 VGPRs, a few spilled) and launched as
 // exactly the blocks that fit: unconstrained it takes 204 VGPRs and runs 2
 // waves per SIMD, 9% slower on the Cornell box.
+// CAP: the variant that records the first frame's camera hits and first
+// shadow rays for the parity tests.
+template <bool CAP>
 __global__ __launch_bounds__(256, 3) void k_shade_bdpt(BdptShadeArgs a, DevCapture cap)
 {
     __shared__ BlockReserveLds<3, 256> rsv;
@@ -437,7 +441,7 @@ __global__ __launch_bounds__(256, 3) void k_shade_bdpt(BdptShadeArgs a, DevCaptu
         int p = -1;
         if (i < n) {
             p = (int)a.list_in[i];
-            o = advance_bdpt(a, cap, p);
+            o = advance_bdpt<CAP>(a, cap, p);
         }
         const bool more = o.ext || o.emit != 0;
         const uint32_t cnt[3] = {o.ext ? 1u : 0u, (uint32_t)__popcll(o.emit), more ? 1u : 0u};
@@ -561,7 +565,8 @@ void launch_generate_bdpt(const GenArgs& a, const DevCapture& cap, hipStream_t s
 
 void launch_shade_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shade_bdpt, dim3(grid), dim3(256), 0, s, a, cap);
+    if (cap.prim_tri) hipLaunchKernelGGL(k_shade_bdpt<true>, dim3(grid), dim3(256), 0, s, a, cap);
+    else hipLaunchKernelGGL(k_shade_bdpt<false>, dim3(grid), dim3(256), 0, s, a, cap);
 }
 
 }  // namespace yune
