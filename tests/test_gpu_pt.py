@@ -393,14 +393,15 @@ def test_deep_tree_uses_stack_overflow_area(width):
     assert (gtri == 0).all()
 
 
-def test_regeneration_with_work_beyond_capacity():
-    """16 frames of 64x64 through 4096 paths in flight: ended paths' slots
-    are regenerated from the remaining work items, and the image is still
-    the oracle's bit for bit."""
+@pytest.mark.parametrize("graph", [0, 1])
+def test_regeneration_with_work_beyond_capacity(graph):
+    """16 frames of 64x64 through 4096 paths in flight: new paths are
+    appended behind the queued rays as room frees up, the bounce loop also
+    replayed from a graph, and the image is still the oracle's bit for bit."""
     s = Y.gen_cornell()
     W, H, F = 64, 64, 16
     kw = dict(mis=1, max_bounces=8, rr_threshold=2)
-    with Y.Context(max_paths=1 << 12, batch_factor=2) as c:
+    with Y.Context(max_paths=1 << 12, batch_factor=2, graph=graph) as c:
         job = _cornell_job(c, s, W, H, Y.params_reference(**kw), 21)
         job.render(F)
         gimg = job.read()

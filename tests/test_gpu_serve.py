@@ -63,12 +63,13 @@ def _jobs(scenes):
     return out
 
 
-@pytest.mark.parametrize("serve_mode", [0, 1])
-def test_served_jobs_equal_jobs_alone(serve_mode):
+@pytest.mark.parametrize("serve_mode,graph", [(0, 0), (1, 0), (0, 1)])
+def test_served_jobs_equal_jobs_alone(serve_mode, graph):
     """serve_mode 0: continuous batching (the bidirectional job runs in a
-    round of its own); 1: scheduler rounds."""
+    round of its own); 1: scheduler rounds; graph 1: the bounce loop
+    replayed from a captured graph."""
     scenes = [Y.gen_cornell(), Y.gen_icosphere_room(3, 9, 2)]
-    with Y.Context(max_paths=1 << 15, serve_mode=serve_mode) as c:
+    with Y.Context(max_paths=1 << 15, serve_mode=serve_mode, graph=graph) as c:
         sids = [c.scene(s) for s in scenes]
         specs = _jobs(scenes)
         jobs = [c.submit(sids[j["sid"]], scenes[j["sid"]].cam, j["W"], j["H"], j["p"], j["frames"], j["seed"])

@@ -524,6 +524,34 @@ int enqueue_iter(yn_ctx* c, const LoopCfg& L, const IterArgs& A, const DevCaptur
     return 0;
 }
 
+// The graph of one group of `group` bounce iterations (starting at queue
+// parity 0), captured once and replayed while the launch arguments stay
+// byte-identical (same buffers, scenes, launch shapes).  The iterations
+// clear their counters with kernels, so the graph holds no memset node.
+int ensure_group_graph(yn_ctx* c, const LoopCfg& L, const IterArgs (&iters)[2], int group)
+{
+    std::vector<uint8_t> key(sizeof iters);
+    std::memcpy(key.data(), iters, sizeof iters);
+    const int32_t shape[] = {L.bdpt, L.width, L.max_depth, L.block, L.grid, L.shade_grid, L.count,
+                             c->cfg.lds_stack, c->cfg.shade_waves, L.overlap, group};
+    key.insert(key.end(), (const uint8_t*)shape, (const uint8_t*)shape + sizeof shape);
+    if (c->gexec && key == c->gkey) return 0;
+    if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    c->gkey.clear();
+    hipStream_t s = c->stream;
+    HIPCK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int r = 0;
+    for (int g = 0; g < group && !r; ++g) r = enqueue_iter(c, L, iters[g & 1], DevCapture{}, nullptr);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (!r && e == hipSuccess) e = hipGraphInstantiateWithFlags(&c->gexec, graph, 0);
+    if (graph) hipGraphDestroy(graph);
+    if (r) return r;
+    if (e != hipSuccess) { c->gexec = nullptr; return fail(c, YN_ERR_HIP, std::string("bounce-loop graph: ") + hipGetErrorString(e)); }
+    c->gkey = std::move(key);
+    return 0;
+}
+
 // Runs one batch of primary-ray regions to completion; samples land in
 // samples[slot][pixel] of each region's job (the sample buffer is shared:
 // job slots are laid out by the caller so regions do not overlap).
@@ -596,33 +624,11 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
     make_iter(c, L, 0, iters[0]);
     make_iter(c, L, 1, iters[1]);
     const int group = 4;                 // iterations between host checks (even)
-    // Graph replay of a group: the launches are captured once and replayed
-    // while their arguments stay byte-identical (same buffers, scenes, launch
-    // shapes). Not with regeneration (the free list alternates between two
-    // buffers) nor with ray capture for tests.
+    // Graph replay of a group (not with ray capture for tests)
     const bool any_cap = cap.prim_tri || cap.prim_light || cap.prim_t || cap.shadow_o || cap.shadow_d || cap.shadow_occ;
-    const bool use_graph = c->cfg.graph != 0 && !regen && !any_cap;
-    if (use_graph) {
-        std::vector<uint8_t> key(sizeof iters);
-        std::memcpy(key.data(), iters, sizeof iters);
-        const int32_t shape[] = {L.bdpt, L.width, L.max_depth, L.block, L.grid, L.shade_grid, L.count,
-                                 c->cfg.lds_stack, c->cfg.shade_waves, L.overlap, group};
-        key.insert(key.end(), (const uint8_t*)shape, (const uint8_t*)shape + sizeof shape);
-        if (!c->gexec || key != c->gkey) {
-            if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-            c->gkey.clear();
-            HIPCK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            int r = 0;
-            for (int g = 0; g < group && !r; ++g) r = enqueue_iter(c, L, iters[g & 1], cap, nullptr);
-            hipGraph_t graph = nullptr;
-            hipError_t e = hipStreamEndCapture(s, &graph);
-            if (!r && e == hipSuccess) e = hipGraphInstantiateWithFlags(&c->gexec, graph, 0);
-            if (graph) hipGraphDestroy(graph);
-            if (r) return r;
-            if (e != hipSuccess) { c->gexec = nullptr; return fail(c, YN_ERR_HIP, std::string("bounce-loop graph: ") + hipGetErrorString(e)); }
-            c->gkey = std::move(key);
-        }
-    }
+    const bool use_graph = c->cfg.graph != 0 && !any_cap;
+    if (use_graph)
+        if (int r = ensure_group_graph(c, L, iters, group)) return r;
 
     int it = 0;
     for (;;) {
@@ -1137,8 +1143,14 @@ int stream_group(yn_ctx* c, bool plan, bool* ran)
     c->serve_rec.st.groups++;
     c->serve_rec.st.paths_in_flight += (uint64_t)S.live;
     c->serve_rec.st.ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
-    for (int g = 0; g < 4; ++g, ++S.it)
-        if (int r = enqueue_iter(c, L, iters[S.it & 1], DevCapture{}, &spans)) return r;
+    if (c->cfg.graph) {     // a group starts at queue parity 0 (four iterations per group)
+        if (int r = ensure_group_graph(c, L, iters, 4)) return r;
+        HIPCK(c, hipGraphLaunch(c->gexec, c->stream));
+        S.it += 4;
+    } else {
+        for (int g = 0; g < 4; ++g, ++S.it)
+            if (int r = enqueue_iter(c, L, iters[S.it & 1], DevCapture{}, &spans)) return r;
+    }
     hipStream_t s = c->stream;
     HIPCK(c, hipGetLastError());
     HIPCK(c, hipMemcpyAsync(c->h_ctr, c->ctr, kCtrHostBytes, hipMemcpyDeviceToHost, s));
