@@ -19,8 +19,9 @@ line carries:
                 roofline rows of its kernels
   batch         64 jobs on 8 icosphere scenes (20k-2M triangles), random
                 cameras, 128^2-1024^2, 16-256 spp, all submitted at once to
-                the continuous-batching scheduler, against the plain per-job
-                paths on the same jobs
+                the continuous-batching scheduler, launch configuration from
+                the library's autotuner, against the plain per-job paths on
+                the same jobs
   serving       the same job mix arriving as a Poisson stream, 20 jobs/s for
                 60 s: jobs/s, p50/p99 latency, against the plain per-job path
 
@@ -146,10 +147,14 @@ def even_sample(n, k):
     return sorted({int(i * n / k) for i in range(k)})
 
 
-def serve_run(Y, np, serve, rate, duration, n_jobs, plain_jobs, frame_jobs, serve_mode=0, graph=0, trace_overlap=0):
+def serve_run(Y, np, serve, rate, duration, n_jobs, plain_jobs, frame_jobs, serve_mode=0, graph=0, trace_overlap=0,
+              tune=True):
     """Served (through the scheduler) against the plain per-job paths on
     jobs of the same list, at equal spp and estimator (the images are
-    identical).  serve False: all jobs submitted at t = 0 (batched)."""
+    identical).  serve False: all jobs submitted at t = 0 (batched).  tune:
+    the library's autotuner picks the launch configuration first, on the
+    list's first 512x512 job at twice the paths in flight (both sides then
+    run with it)."""
     scenes = serving_scenes(Y, np)
     n_jobs = n_jobs or (int(duration * rate) if serve else 64)
     jobs = serving_jobs(Y, np, len(scenes), n_jobs, rate if serve else 0.0)
@@ -159,6 +164,17 @@ def serve_run(Y, np, serve, rate, duration, n_jobs, plain_jobs, frame_jobs, serv
         sids = [ctx.scene(s) for s in scenes]
         print(f"# {len(scenes)} scenes, {sum(len(s.tris) for s in scenes)} triangles, built and uploaded in "
               f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        tuned = None
+        if tune:
+            t0 = time.perf_counter()
+            j = next((j for j in jobs if j["W"] == 512), jobs[0])
+            frames = max(2, 2 * ctx.config()["batch_paths"] // (j["W"] * j["H"]))
+            tuned = ctx.autotune(sids[j["scene"]], j["cam"], j["W"], j["H"], j["params"], frames=frames)
+            print(f"# autotuned in {time.perf_counter() - t0:.1f}s: {tuned}", file=sys.stderr, flush=True)
+        launch = {k: v for k, v in ctx.config().items()
+                  if k in ("max_paths", "bvh_width", "lds_stack", "shade_waves", "trace_block", "trace_chunk",
+                           "trace_xcd", "trace_refill", "graph", "trace_overlap", "batch_factor", "batch_paths",
+                           "serve_mode")}
         # warm the kernels once
         w = ctx.job(sids[0], jobs[0]["cam"], 64, 64, jobs[0]["params"])
         w.render(1)
@@ -241,6 +257,7 @@ def serve_run(Y, np, serve, rate, duration, n_jobs, plain_jobs, frame_jobs, serv
                     "NEE+MIS, 64 GB image budget, %s" % ("continuous batching" if serve_mode == 0 else "rounds"),
         "jobs": len(jobs), "scene_tris": [len(s.tris) for s in scenes],
         "total_megasamples": round(samples / 1e6, 1),
+        "launch": launch, "autotuned": tuned is not None,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1), "p99": round(float(np.percentile(lat, 99)), 1),
                        "max": round(float(lat.max()), 1)},
         "makespan_s": round(makespan, 3),
@@ -529,6 +546,7 @@ def main():
     ap.add_argument("--bvh-build", type=int, default=0, help="GPU tree: 1 own SAH build, 2 the reference's tree")
     ap.add_argument("--hoist-frac", type=float, default=0.0, help="box-area fraction above which triangles are hoisted")
     ap.add_argument("--serve-mode", type=int, default=0, help="batch / serve: 0 continuous batching, 1 rounds")
+    ap.add_argument("--no-tune", action="store_true", help="batch / serve: library defaults instead of the autotuner")
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to calibrate PMC counters")
     args = ap.parse_args()
     if args.copy_probe:
@@ -547,7 +565,8 @@ def main():
         serve = kind == "serve"
         plain = args.plain_jobs if args.plain_jobs >= 0 else (120 if serve else 0)
         return serve_run(Y, np, serve, args.rate, args.duration, args.jobs, plain, args.frame_jobs,
-                         serve_mode=args.serve_mode, graph=args.graph, trace_overlap=max(args.trace_overlap, 0))
+                         serve_mode=args.serve_mode, graph=args.graph, trace_overlap=max(args.trace_overlap, 0),
+                         tune=not args.no_tune)
 
     if args.workload in ("batch", "serve"):
         r = serving(args.workload)

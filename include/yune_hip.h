@@ -60,8 +60,8 @@ typedef struct yn_config {
     int32_t shade_waves;      /* occupancy the shade kernel is compiled for: 2, 3, 4
                                  (0: 4)                                             */
     int32_t batch_factor;     /* path-tracing work per batch over paths in flight:
-                                 1 (0: default) or 2, when ended paths' slots are
-                                 regenerated from the remaining work               */
+                                 1 (0: default) or 2: the rest enters as paths end,
+                                 behind the rays already queued (regeneration)     */
     int32_t trace_overlap;    /* 1: each iteration's closest-hit and any-hit traces
                                  run on two streams and overlap; 0: one stream     */
     int32_t graph;            /* 1: groups of bounce iterations are captured once in
@@ -255,8 +255,8 @@ double yn_clock_ms(yn_ctx* ctx);                     /* ms since yn_create */
 typedef struct yn_tune_result {
     int32_t bvh_width, trace_block, trace_chunk, lds_stack, shade_waves, n_tried;
     int64_t batch_paths;      /* paths in flight per batch                              */
-    int32_t batch_factor;     /* work items per batch over paths in flight; 2: ended
-                                 paths' slots are regenerated from the remaining work */
+    int32_t batch_factor;     /* work items per batch over paths in flight; 2: the
+                                 rest enters as paths end (regeneration)             */
     int32_t trace_overlap;    /* closest-hit and any-hit traces on two streams        */
     int32_t graph;            /* bounce loop replayed from a HIP graph                */
     int32_t trace_xcd;        /* ray-queue slices per trace launch                    */

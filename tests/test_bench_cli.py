@@ -105,6 +105,12 @@ def test_serve_bookkeeping_with_a_stand_in_library():
         def serve(self, n):
             return 0
 
+        def autotune(self, *a, **k):
+            return {"trace_block": 256, "mrays_per_s": 1.0}
+
+        def config(self):
+            return {"max_paths": 4, "trace_block": 256, "graph": 0, "unrelated": 1, "batch_paths": 4}
+
         def serve_stats(self):
             return types.SimpleNamespace(rays_ext=10, rays_shadow=5, ms_trace_ext=1.0, ms_trace_shadow=1.0,
                                          ms_shade=1.0, ms_host=0.1, groups=1, refills=1, plan_stalls=0,
@@ -115,5 +121,7 @@ def test_serve_bookkeeping_with_a_stand_in_library():
     r = bench.serve_run(Y, np, True, 20.0, 1.0, 0, 5, 2)
     assert r["jobs"] == 20 and r["plain"]["job_at_a_time_jobs"] == 5 and r["plain"]["frame_at_a_time_jobs"] == 2
     assert r["latency_ms"]["p99"] >= r["latency_ms"]["p50"] and r["plain"]["images_identical_to_served"]
-    r = bench.serve_run(Y, np, False, 20.0, 1.0, 0, 0, 2)
+    assert r["autotuned"] and r["launch"] == {"max_paths": 4, "trace_block": 256, "graph": 0, "batch_paths": 4}
+    r = bench.serve_run(Y, np, False, 20.0, 1.0, 0, 0, 2, tune=False)
+    assert not r["autotuned"]
     assert r["jobs"] == 64 and r["plain"]["job_at_a_time_jobs"] == 64
