@@ -1193,6 +1193,26 @@ int stream_drain(yn_ctx* c)
 
 // GPU layout of scene `sid` at `width` from its kept host arrays, uploaded
 // in place of the previous one (the scene id stays valid).
+// Whether every present child box of the laid-out tree has finite
+// coordinates within 2^60, the bound under which the traversal's FMA slab
+// form cannot overflow (boxes of absent children are never tested).
+static bool fma_slab_ok(const GpuLayout& L)
+{
+    const int w = L.width, fpn = w == 2 ? 16 : 32;
+    for (int n = 0; n < L.n_nodes; ++n) {
+        const float* g = &L.nodes[(size_t)n * fpn];
+        const int32_t* ref = reinterpret_cast<const int32_t*>(g + (w == 2 ? 12 : 24));
+        for (int k = 0; k < w; ++k) {
+            if ((uint32_t)ref[k] == kEmptyRef) continue;
+            for (int a = 0; a < 3; ++a) {
+                const float lo = g[2 * w * a + k], hi = g[2 * w * a + w + k];
+                if (!(std::fabs(lo) <= 0x1p60f) || !(std::fabs(hi) <= 0x1p60f)) return false;
+            }
+        }
+    }
+    return true;
+}
+
 int build_scene(yn_ctx* c, int sid, int width)
 {
     SceneRec& S = c->scenes[sid];
@@ -1249,6 +1269,7 @@ int build_scene(yn_ctx* c, int sid, int width)
     D.brute = L.brute;
     D.n_slots = (int32_t)L.slot_tri.size();
     D.n_hoist = L.n_hoist;
+    D.fma_slab = fma_slab_ok(L) ? 1 : 0;
     HIPCK(c, hipMemcpy(c->d_scenes + sid, &D, sizeof(DevScene), hipMemcpyHostToDevice));
     S.info.n_tri = (int32_t)S.h_tris.size();
     S.info.n_ref_nodes = (int32_t)S.h_nodes.size();

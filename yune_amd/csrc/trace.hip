@@ -35,54 +35,113 @@ namespace {
 
 // One slab of rayAabbIntersection (udpt.cl:397-418).  min_diff can only be
 // NaN when dir_inv is infinite (a zero or denormal direction component) and
-// the plane passes through the origin; EXACT keeps the reference's NaN skip,
-// the other form is used for waves whose rays all have finite dir_inv, where
-// the two agree bit for bit.
-template <bool EXACT>
+// the plane passes through the origin, where the reference skips the slab.
 __device__ __forceinline__ void slab(float lo, float hi, float o, float inv, float& tmin, float& tmax)
 {
     float a = (lo - o) * inv;           // min_diff (udpt.cl:397)
     float b = (hi - o) * inv;           // max_diff (udpt.cl:398)
-    if (!EXACT || !(a != a)) {          // !isnan(min_diff)
+    if (!(a != a)) {                    // !isnan(min_diff)
         tmin = fmaxf(fminf(a, b), tmin);
         tmax = fminf(fmaxf(a, b), tmax);
     }
 }
 
 // rayAabbIntersection (udpt.cl:392-431) + entry distance for culling.
-template <bool EXACT = true>
 __device__ __forceinline__ bool box_hit(float lox, float loy, float loz, float hix, float hiy, float hiz,
                                         float ox, float oy, float oz, float ix, float iy, float iz,
                                         float& enter)
 {
-    float tmin, tmax;
-    if (EXACT) {
-        tmin = -INFINITY;
-        tmax = INFINITY;
-        slab<EXACT>(lox, hix, ox, ix, tmin, tmax);
-        slab<EXACT>(loy, hiy, oy, iy, tmin, tmax);
-        slab<EXACT>(loz, hiz, oz, iz, tmin, tmax);
-    } else {
-        // no NaN can occur: the same maxima and minima without the
-        // reference's -inf / +inf starting values (one min per box fewer)
-        const float ax = (lox - ox) * ix, bx = (hix - ox) * ix;
-        const float ay = (loy - oy) * iy, by = (hiy - oy) * iy;
-        const float az = (loz - oz) * iz, bz = (hiz - oz) * iz;
-        tmin = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
-        tmax = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
-    }
+    float tmin = -INFINITY, tmax = INFINITY;
+    slab(lox, hix, ox, ix, tmin, tmax);
+    slab(loy, hiy, oy, iy, tmin, tmax);
+    slab(loz, hiz, oz, iz, tmin, tmax);
     enter = fmaxf(tmin, 0.0f);
     return tmax > enter;
 }
 
 struct RayState {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+    float lx, ly, lz;       // FMA slab form: o * dir_inv moved by 2^-21 |o * dir_inv|,
+    float hx, hy, hz;       //   one way for the box's lo planes, the other for its hi planes
     float best;             // current tmax
-    float cull;             // best * (1 + 1e-4)
-    int tri;
-    float u, v;
-    int overflow;           // traversal stack would have overflowed
+    float cull;             // nodes entered beyond it are skipped: best * (1 + 1e-4) * kWiden
+    int tri;                // best hit: triangle id (-1: none), u, v; for any-hit rays
+    float u, v;             //   the slot, with id, u and v recomputed at the end if wanted
 };
+
+constexpr float kWiden = 1.0f + 0x1p-19f;   // relative widening of the FMA slab form
+
+// The cull distance for a best hit t: the margin 1e-4 the traversal always
+// had, widened as the FMA slab form's exit distance is (below), so that a
+// node the reference's arithmetic would enter is never culled by it.
+__device__ __forceinline__ float cull_of(float t) { return t * 1.0001f * kWiden; }
+
+// The slab test with o * dir_inv precomputed per ray: each slab distance is
+// one FMA, plane * dir_inv - o * dir_inv, where the reference takes a
+// difference and a product (u = 2^-24 below).  Rounding o * dir_inv costs
+// up to u|o * dir_inv| in absolute terms; the offsets subtracted are moved
+// by 8u|o * dir_inv| (2^-21, exact), towards smaller distances for the
+// planes the ray enters through and larger ones for those it leaves
+// through (which of lo and hi that is follows the sign of dir_inv), which
+// covers that error and the rounding of the move.  What remains is
+// relative: each entry distance is at most 4u of its own size above the
+// reference's, each exit distance at most 4u below; min and max are
+// monotone, so the same holds for the box's entry and exit.  The exit
+// distance is therefore widened by kWiden = 1 + 32u before the comparison,
+// with >= where the reference has >, and cull distances likewise: every box
+// the reference's arithmetic enters is entered here.  The triangle tests
+// are exact and the closest hit is the least t over the triangles tested,
+// so the few boxes visited beyond the reference's change a result only
+// where its arithmetic drops a box that a triangle hit touches to within
+// rounding (a hit on the box's boundary), as a tie in t would.  Used for
+// rays with |dir_inv| and |o| at most 2^40 in scenes whose box coordinates
+// are finite and at most 2^60, where nothing can overflow; other rays take
+// box_hit.
+__device__ __forceinline__ bool box_hit_fma(float lox, float loy, float loz, float hix, float hiy, float hiz,
+                                            const RayState& r, float& enter)
+{
+    const float ax = fmaf(lox, r.ix, -r.lx), bx = fmaf(hix, r.ix, -r.hx);
+    const float ay = fmaf(loy, r.iy, -r.ly), by = fmaf(hiy, r.iy, -r.hy);
+    const float az = fmaf(loz, r.iz, -r.lz), bz = fmaf(hiz, r.iz, -r.hz);
+    const float tmin = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+    const float tmax = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    enter = fmaxf(tmin, 0.0f);
+    return tmax * kWiden >= enter;
+}
+
+// A child box of an interior node: the reference's slab arithmetic (EXACT)
+// or the widened FMA form.
+template <bool EXACT>
+__device__ __forceinline__ bool child_hit(float lox, float loy, float loz, float hix, float hiy, float hiz,
+                                          const RayState& r, float& enter)
+{
+    return EXACT ? box_hit(lox, loy, loz, hix, hiy, hiz, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, enter)
+                 : box_hit_fma(lox, loy, loz, hix, hiy, hiz, r, enter);
+}
+
+// A ray's traversal state from its queue entry.  Returns whether the FMA
+// slab form may be used (see box_hit_fma).
+__device__ __forceinline__ bool start_ray(RayState& r, const float4& ot, const float4& dp, const DevScene& S)
+{
+    r.ox = ot.x; r.oy = ot.y; r.oz = ot.z;
+    r.dx = dp.x; r.dy = dp.y; r.dz = dp.z;
+    r.ix = 1 / r.dx; r.iy = 1 / r.dy; r.iz = 1 / r.dz;   // dir_inv (udpt.cl:395)
+    auto offsets = [](float o, float inv, float& l, float& h) {
+        const float oi = o * inv, e = fabsf(oi) * 0x1p-21f;
+        // entering through lo when dir_inv > 0, through hi when < 0
+        l = inv > 0.0f ? oi + e : oi - e;
+        h = inv > 0.0f ? oi - e : oi + e;
+    };
+    offsets(r.ox, r.ix, r.lx, r.hx);
+    offsets(r.oy, r.iy, r.ly, r.hy);
+    offsets(r.oz, r.iz, r.lz, r.hz);
+    r.best = ot.w;
+    r.cull = cull_of(ot.w);
+    r.tri = -1; r.u = 0.0f; r.v = 0.0f;
+    const float lim = 0x1p40f;
+    return S.fma_slab && fabsf(r.ix) <= lim && fabsf(r.iy) <= lim && fabsf(r.iz) <= lim && fabsf(r.ox) <= lim &&
+           fabsf(r.oy) <= lim && fabsf(r.oz) <= lim;
+}
 
 // A load from a scene array as a global-memory load.  Through a generic
 // pointer (the scene table holds them) the compiler emits FLAT loads, which
@@ -106,10 +165,13 @@ __device__ __forceinline__ int4 gload(const int4* p)
 }
 
 // rayTriangleIntersection (udpt.cl:326-390) on a stored slot; xyz only
-// (the w terms of the reference's dots are exact zeros).
-__device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int s, RayState& r)
+// (the w terms of the reference's dots are exact zeros).  Returns whether
+// (u, v) lies in the triangle, with t; id: the triangle id.
+__device__ __forceinline__ bool tri_uvt(const float4* __restrict__ tri, int s, const RayState& r, float& t, float& u,
+                                        float& v, int& id)
 {
     float4 A = gload(tri + 3 * s), B = gload(tri + 3 * s + 1), C = gload(tri + 3 * s + 2);
+    id = __float_as_int(A.w);
     // pvec = cross(dir, v1v3)
     float px = r.dy * C.z - r.dz * C.y;
     float py = r.dz * C.x - r.dx * C.z;
@@ -117,24 +179,49 @@ __device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int s, 
     float det = B.x * px + B.y * py + B.z * pz;
     float inv_det = 1.0f / det;
     float sx = r.ox - A.x, sy = r.oy - A.y, sz = r.oz - A.z;
-    float u = (px * sx + py * sy + pz * sz) * inv_det;
+    u = (px * sx + py * sy + pz * sz) * inv_det;
     if (u < 0.0f || u > 1.0f) return false;
     // qvec = cross(dist, v1v2)
     float qx = sy * B.z - sz * B.y;
     float qy = sz * B.x - sx * B.z;
     float qz = sx * B.y - sy * B.x;
-    float v = (qx * r.dx + qy * r.dy + qz * r.dz) * inv_det;
+    v = (qx * r.dx + qy * r.dy + qz * r.dz) * inv_det;
     if (v < 0.0f || u + v > 1.0f) return false;
-    float t = (C.x * qx + C.y * qy + C.z * qz) * inv_det;
-    if (t > 0.0f && t < r.best) {
+    t = (C.x * qx + C.y * qy + C.z * qz) * inv_det;
+    return true;
+}
+
+template <bool ANY>
+__device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int s, RayState& r)
+{
+    float t, u, v;
+    int id;
+    if (tri_uvt(tri, s, r, t, u, v, id) && t > 0.0f && t < r.best) {
         r.best = t;
-        r.cull = t * 1.0001f;
-        r.tri = __float_as_int(A.w);
-        r.u = u;
-        r.v = v;
+        r.cull = cull_of(t);
+        if (ANY) {
+            r.tri = s;
+        } else {
+            r.tri = id;
+            r.u = u;
+            r.v = v;
+        }
         return true;
     }
     return false;
+}
+
+// The result record {t, triangle id bits, u, v} of a finished traversal.
+// Any-hit rays kept the slot: its id, u and v come from the same test on
+// the same inputs as when it was found.
+template <bool ANY>
+__device__ __forceinline__ float4 hit_record(const float4* __restrict__ tri, const RayState& r)
+{
+    if (!ANY || r.tri < 0) return make_float4(r.best, __int_as_float(r.tri), r.u, r.v);
+    float t, u, v;
+    int id;
+    tri_uvt(tri, r.tri, r, t, u, v, id);
+    return make_float4(r.best, __int_as_float(id), u, v);
 }
 
 struct Counts { unsigned nodes, tris; };
@@ -147,7 +234,7 @@ __device__ __forceinline__ bool leaf_test(const float4* __restrict__ tri, uint32
     bool got = false;
     for (int k = 0; k < cnt; ++k) {
         if (COUNT) c.tris++;
-        got |= tri_test(tri, first + k, r);
+        got |= tri_test<ANY>(tri, first + k, r);
         if (ANY && got) return true;
     }
     return got;
@@ -162,6 +249,7 @@ struct Stack {
     uint16_t* gt;
     uint32_t gstride;       // threads in the grid
     int gcap;               // overflow entries available
+    uint32_t* full;         // set to 1 by a traversal that would overflow the stack (it ends there)
     int sp;
 
     __device__ __forceinline__ bool push(uint32_t ref, float t)
@@ -207,7 +295,7 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
     if (S.brute) {
         for (int s = 0; s < S.n_slots; ++s) {
             if (COUNT) c.tris++;
-            got |= tri_test(tri, s, r);
+            got |= tri_test<ANY>(tri, s, r);
             if (ANY && got) return true;
         }
         return got;
@@ -219,7 +307,7 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
         if (!box_hit(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e) || e > r.cull)
             continue;
         if (COUNT) c.tris++;
-        got |= tri_test(tri, s, r);
+        got |= tri_test<ANY>(tri, s, r);
         if (ANY && got) return true;
     }
     uint32_t node = S.root_ref;
@@ -239,15 +327,15 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
                 float4 X = gload(n), Y = gload(n + 1), Z = gload(n + 2);
                 int4 C = gload(reinterpret_cast<const int4*>(n + 3));
                 float e0, e1;
-                bool h0 = box_hit<EXACT>(X.x, Y.x, Z.x, X.z, Y.z, Z.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e0);
-                bool h1 = box_hit<EXACT>(X.y, Y.y, Z.y, X.w, Y.w, Z.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e1);
+                bool h0 = child_hit<EXACT>(X.x, Y.x, Z.x, X.z, Y.z, Z.z, r, e0);
+                bool h1 = child_hit<EXACT>(X.y, Y.y, Z.y, X.w, Y.w, Z.w, r, e1);
                 h0 = h0 && e0 <= r.cull && (uint32_t)C.x != kEmptyRef;
                 h1 = h1 && e1 <= r.cull && (uint32_t)C.y != kEmptyRef;
                 if (h0 && h1) {
                     uint32_t nearc = (uint32_t)C.x, farc = (uint32_t)C.y;
                     float fe = e1;
                     if (e1 < e0) { nearc = (uint32_t)C.y; farc = (uint32_t)C.x; fe = e0; }
-                    if (!st.push(farc, fe)) { r.overflow = 1; return got; }   // cannot happen: depth bound
+                    if (!st.push(farc, fe)) { atomicOr(st.full, 1u); return got; }   // cannot happen: depth bound
                     node = nearc;
                     continue;
                 } else if (h0) { node = (uint32_t)C.x; continue; }
@@ -257,10 +345,10 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
                 float4 LX = gload(n), HX = gload(n + 1), LY = gload(n + 2), HY = gload(n + 3), LZ = gload(n + 4), HZ = gload(n + 5);
                 int4 C = gload(reinterpret_cast<const int4*>(n + 6));
                 float k0, k1, k2, k3;
-                bool h0 = box_hit<EXACT>(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k0);
-                bool h1 = box_hit<EXACT>(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k1);
-                bool h2 = box_hit<EXACT>(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k2);
-                bool h3 = box_hit<EXACT>(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k3);
+                bool h0 = child_hit<EXACT>(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r, k0);
+                bool h1 = child_hit<EXACT>(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r, k1);
+                bool h2 = child_hit<EXACT>(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r, k2);
+                bool h3 = child_hit<EXACT>(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r, k3);
                 uint32_t c0 = (uint32_t)C.x, c1 = (uint32_t)C.y, c2 = (uint32_t)C.z, c3 = (uint32_t)C.w;
                 // missed, culled or absent children sort last (key +inf)
                 k0 = (h0 && k0 <= r.cull && c0 != kEmptyRef) ? k0 : INFINITY;
@@ -285,7 +373,7 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
                     if (k3 != INFINITY) ok &= st.push(c3, k3);
                     if (k2 != INFINITY) ok &= st.push(c2, k2);
                     if (k1 != INFINITY) ok &= st.push(c1, k1);
-                    if (!ok) { r.overflow = 1; return got; }   // cannot happen: depth bound
+                    if (!ok) { atomicOr(st.full, 1u); return got; }   // cannot happen: depth bound
                     node = c0;
                     continue;
                 }
@@ -326,15 +414,15 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, cons
             float4 X = gload(n), Y = gload(n + 1), Z = gload(n + 2);
             int4 C = gload(reinterpret_cast<const int4*>(n + 3));
             float e0, e1;
-            bool h0 = box_hit<EXACT>(X.x, Y.x, Z.x, X.z, Y.z, Z.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e0);
-            bool h1 = box_hit<EXACT>(X.y, Y.y, Z.y, X.w, Y.w, Z.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e1);
+            bool h0 = child_hit<EXACT>(X.x, Y.x, Z.x, X.z, Y.z, Z.z, r, e0);
+            bool h1 = child_hit<EXACT>(X.y, Y.y, Z.y, X.w, Y.w, Z.w, r, e1);
             h0 = h0 && e0 <= r.cull && (uint32_t)C.x != kEmptyRef;
             h1 = h1 && e1 <= r.cull && (uint32_t)C.y != kEmptyRef;
             if (h0 && h1) {
                 uint32_t nearc = (uint32_t)C.x, farc = (uint32_t)C.y;
                 float fe = e1;
                 if (e1 < e0) { nearc = (uint32_t)C.y; farc = (uint32_t)C.x; fe = e0; }
-                if (!st.push(farc, fe)) { r.overflow = 1; return false; }   // cannot happen: depth bound
+                if (!st.push(farc, fe)) { atomicOr(st.full, 1u); return false; }   // cannot happen: depth bound
                 node = nearc;
                 if (!(node & kLeafBit)) return true;
             } else if (h0) { node = (uint32_t)C.x; if (!(node & kLeafBit)) return true; }
@@ -345,10 +433,10 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, cons
             float4 LX = gload(n), HX = gload(n + 1), LY = gload(n + 2), HY = gload(n + 3), LZ = gload(n + 4), HZ = gload(n + 5);
             int4 C = gload(reinterpret_cast<const int4*>(n + 6));
             float k0, k1, k2, k3;
-            bool h0 = box_hit<EXACT>(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k0);
-            bool h1 = box_hit<EXACT>(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k1);
-            bool h2 = box_hit<EXACT>(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k2);
-            bool h3 = box_hit<EXACT>(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, k3);
+            bool h0 = child_hit<EXACT>(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r, k0);
+            bool h1 = child_hit<EXACT>(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r, k1);
+            bool h2 = child_hit<EXACT>(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r, k2);
+            bool h3 = child_hit<EXACT>(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r, k3);
             uint32_t c0 = (uint32_t)C.x, c1 = (uint32_t)C.y, c2 = (uint32_t)C.z, c3 = (uint32_t)C.w;
             // missed, culled or absent children sort last (key +inf)
             k0 = (h0 && k0 <= r.cull && c0 != kEmptyRef) ? k0 : INFINITY;
@@ -373,7 +461,7 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, cons
                 if (k3 != INFINITY) ok &= st.push(c3, k3);
                 if (k2 != INFINITY) ok &= st.push(c2, k2);
                 if (k1 != INFINITY) ok &= st.push(c1, k1);
-                if (!ok) { r.overflow = 1; return false; }   // cannot happen: depth bound
+                if (!ok) { atomicOr(st.full, 1u); return false; }   // cannot happen: depth bound
                 node = c0;
                 if (!(node & kLeafBit)) return true;
             } else goto pop;
@@ -417,6 +505,7 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
     st.gref = a.ovf_ref + gtid;
     st.gt = a.ovf_t + gtid;
     st.gcap = a.ovf_depth;
+    st.full = &a.ctr->q[7][0];
     const int lane = (int)__lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.zero0) { *a.zero0 = 0; if (a.zero1) *a.zero1 = 0; }
     const uint32_t n = *a.q.count;
@@ -476,16 +565,8 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
             if (a.by_queue) pidx = (int)i;      // the result goes beside the ray
             const DevScene& S = a.scenes[sc];
             RayState r;
-            r.ox = ot.x; r.oy = ot.y; r.oz = ot.z;
-            r.dx = dp.x; r.dy = dp.y; r.dz = dp.z;
-            r.ix = 1 / r.dx; r.iy = 1 / r.dy; r.iz = 1 / r.dz;   // dir_inv (udpt.cl:395)
-            r.best = ot.w;
-            r.cull = ot.w * 1.0001f;
-            r.tri = -1; r.u = 0.0f; r.v = 0.0f; r.overflow = 0;
+            const bool fma_ok = start_ray(r, ot, dp, S);
             st.sp = 0;
-            // only an infinite dir_inv (a zero or denormal direction
-            // component) can make the reference's NaN guard matter
-            bool inf_inv = fabsf(r.ix) == INFINITY || fabsf(r.iy) == INFINITY || fabsf(r.iz) == INFINITY;
             // A ray with a NaN in its origin, direction or length hits nothing:
             // u, v or t of every triangle test is NaN, and each acceptance
             // comparison is false.  Its traversal would still visit every
@@ -495,15 +576,14 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
                              dp.y == dp.y && dp.z == dp.z);
             bool got = false;
             if (!nan_ray)
-                got = __ballot(inf_inv) ? traverse<W, ANY, LST, BLOCK, COUNT, true>(S, r, st, c)
+                got = __ballot(!fma_ok) ? traverse<W, ANY, LST, BLOCK, COUNT, true>(S, r, st, c)
                                         : traverse<W, ANY, LST, BLOCK, COUNT, false>(S, r, st, c);
             ++nrays;
-            if (r.overflow) atomicOr(&a.ctr->q[7][0], 1u);
             if (ANY) {
                 a.occ_out[pidx] = got ? 1 : 0;
-                if (a.hit_out) a.hit_out[pidx] = make_float4(r.best, __int_as_float(got ? r.tri : -1), r.u, r.v);
+                if (a.hit_out) a.hit_out[pidx] = hit_record<ANY>(S.tri_isect, r);
             } else {
-                a.hit_out[pidx] = make_float4(r.best, __int_as_float(r.tri), r.u, r.v);
+                a.hit_out[pidx] = hit_record<ANY>(S.tri_isect, r);
             }
         }
     }
@@ -521,14 +601,14 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
 
 // Result of one ray into the path's slot.
 template <bool ANY>
-__device__ __forceinline__ void store_hit(const TraceArgs& a, int pidx, const RayState& r, bool got)
+__device__ __forceinline__ void store_hit(const TraceArgs& a, int pidx, const float4* __restrict__ tri,
+                                          const RayState& r, bool got)
 {
-    if (r.overflow) atomicOr(&a.ctr->q[7][0], 1u);
     if (ANY) {
         a.occ_out[pidx] = got ? 1 : 0;
-        if (a.hit_out) a.hit_out[pidx] = make_float4(r.best, __int_as_float(got ? r.tri : -1), r.u, r.v);
+        if (a.hit_out) a.hit_out[pidx] = hit_record<ANY>(tri, r);
     } else {
-        a.hit_out[pidx] = make_float4(r.best, __int_as_float(r.tri), r.u, r.v);
+        a.hit_out[pidx] = hit_record<ANY>(tri, r);
     }
 }
 
@@ -553,6 +633,7 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
     st.gref = a.ovf_ref + gtid;
     st.gt = a.ovf_t + gtid;
     st.gcap = a.ovf_depth;
+    st.full = &a.ctr->q[7][0];
     st.sp = 0;
     const int lane = (int)__lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -609,17 +690,10 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
                                              : a.jobs[a.by_queue ? pidx : a.path_job[pidx >> a.id_shift]].scene;
                 if (a.by_queue) pidx = (int)i;      // the result goes beside the ray
                 const DevScene& S = a.scenes[sc];
-                r.ox = ot.x; r.oy = ot.y; r.oz = ot.z;
-                r.dx = dp.x; r.dy = dp.y; r.dz = dp.z;
-                r.ix = 1 / r.dx; r.iy = 1 / r.dy; r.iz = 1 / r.dz;   // dir_inv (udpt.cl:395)
-                r.best = ot.w;
-                r.cull = ot.w * 1.0001f;
-                r.tri = -1; r.u = 0.0f; r.v = 0.0f; r.overflow = 0;
+                exact = !start_ray(r, ot, dp, S);
                 st.sp = 0;
                 got = false;
                 ++nrays;
-                // only an infinite dir_inv can make the reference's NaN guard matter
-                exact = fabsf(r.ix) == INFINITY || fabsf(r.iy) == INFINITY || fabsf(r.iz) == INFINITY;
                 // a ray with a NaN component hits nothing (see k_trace)
                 live = ot.x == ot.x && ot.y == ot.y && ot.z == ot.z && ot.w == ot.w && dp.x == dp.x &&
                        dp.y == dp.y && dp.z == dp.z;
@@ -630,7 +704,7 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
                 if (live && S.brute) {
                     for (int s = 0; s < S.n_slots; ++s) {
                         if (COUNT) c.tris++;
-                        got |= tri_test(tri, s, r);
+                        got |= tri_test<ANY>(tri, s, r);
                         if (ANY && got) break;
                     }
                     live = false;
@@ -643,7 +717,7 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
                             e > r.cull)
                             continue;
                         if (COUNT) c.tris++;
-                        got |= tri_test(tri, s, r);
+                        got |= tri_test<ANY>(tri, s, r);
                         if (ANY && got) break;
                     }
                     node = S.root_ref;
@@ -657,7 +731,7 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
                         live = false;
                     }
                 }
-                if (!live) store_hit<ANY>(a, pidx, r, got);
+                if (!live) store_hit<ANY>(a, pidx, tri, r, got);
             }
         }
         if (__ballot(live) == 0) {
@@ -668,7 +742,7 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
         if (live) {
             live = wexact ? trav_step<W, ANY, LST, BLOCK, COUNT, true>(nodes, tri, node, r, st, got, c)
                           : trav_step<W, ANY, LST, BLOCK, COUNT, false>(nodes, tri, node, r, st, got, c);
-            if (!live) store_hit<ANY>(a, pidx, r, got);
+            if (!live) store_hit<ANY>(a, pidx, tri, r, got);
         }
     }
     unsigned long long tot = wave_sum_u64(nrays);

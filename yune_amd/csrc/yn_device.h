@@ -29,6 +29,8 @@ struct DevScene {
     int32_t brute;              // 1: no BVH, test every slot in order
     int32_t n_slots;
     int32_t n_hoist;            // slots [0, n_hoist) are tested before traversal
+    int32_t fma_slab;           // 1: every child box coordinate is finite and within
+                                //    2^60 (the traversal's FMA slab form cannot overflow)
 };
 
 // Per-job constants (a job = one image being accumulated).
