@@ -390,6 +390,17 @@ def kernel_row(kernel, work_unit, units, launches, ms, summary, key, alg_bytes=N
             row["valu_busy"] = round(vb / 100.0, 3)
         if isinstance(vu, float):
             row["valu_util"] = round(vu / 100.0, 3)
+        # wave-level instructions per unit of work and the texture address /
+        # data units' busy share (the memory pipeline in front of the L1)
+        u = work_unit.split()[-1]
+        for part in ("valu", "salu", "vmem"):
+            x = summary.get(f"{valu_key}_{part}_insts_per_{u}")
+            if isinstance(x, float):
+                row[f"{part}_insts_per_{u}"] = round(x, 2)
+        for part in ("ta", "td"):
+            x = summary.get(f"{valu_key}_{part}_busy")
+            if isinstance(x, float):
+                row[f"{part}_busy"] = round(x / 100.0, 3)
     if summary:
         row["counters_source"] = "profiles/" + summary["source"]
     return row
@@ -472,11 +483,16 @@ def workload_line(Y, np, name, args, with_cpu):
     if "valu_busy" in main_row:
         R["valu_busy"] = main_row["valu_busy"]
         R["valu_util"] = main_row.get("valu_util")
-        # what limits the kernel: the larger of its HBM fraction and its
-        # vector-issue fraction
-        hf = R["frac"] or 0.0
-        R["limiter"] = "vector issue" if main_row["valu_busy"] > hf else "hbm"
-        R["bound_frac"] = round(max(main_row["valu_busy"], hf), 4)
+        for k in ("td_busy", "ta_busy"):
+            if k in main_row:
+                R[k] = main_row[k]
+        # what limits the kernel: the largest of its HBM fraction, its
+        # vector-issue fraction and the busy share of the texture data
+        # unit (the vector-memory pipeline's return path into registers)
+        cands = {"hbm": R["frac"] or 0.0, "vector issue": main_row["valu_busy"],
+                 "vector memory pipeline": main_row.get("td_busy", 0.0)}
+        R["limiter"] = max(cands, key=cands.get)
+        R["bound_frac"] = round(max(cands.values()), 4)
     R["units_per_launch"] = main_row["units_per_launch"]
     R["avg_launch_ms"] = main_row["avg_launch_ms"]
     R["rows"] = rows

@@ -43,7 +43,7 @@ def test_roofline_rows_from_a_summary():
     sys.path.insert(0, ROOT)
     import bench
     summary = {"source": "x.txt", "trace_ext_bytes_per_ray": 800.0, "trace_ext_valu_busy": 70.0,
-               "trace_ext_valu_util": 46.0}
+               "trace_ext_valu_util": 46.0, "trace_ext_valu_insts_per_ray": 150.0, "trace_ext_td_busy": 98.0}
     row = bench.kernel_row("k", "ray", units=16e6 * 10, launches=10, ms=50.0, summary=summary,
                            key="trace_ext_bytes_per_ray", alg_bytes=2500.0, valu_key="trace_ext")
     assert row["units_per_launch"] == 16_000_000 and row["avg_launch_ms"] == 5.0
@@ -51,6 +51,7 @@ def test_roofline_rows_from_a_summary():
     assert abs(row["achieved"] - 800 * 16e6 / 5e-3 / 1e9) < 0.1
     assert abs(row["frac"] - row["achieved"] / 8000.0) < 1e-4
     assert row["algorithmic"]["bytes_per_unit"] == 2500.0 and row["valu_busy"] == 0.7
+    assert row["valu_insts_per_ray"] == 150.0 and row["td_busy"] == 0.98
 
 
 def test_serve_bookkeeping_with_a_stand_in_library():
