@@ -39,6 +39,25 @@
 #define YNF_INV_LN2 1.4426950408889634
 #define YNF_SQRT2 1.4142135623730951
 
+/* Polynomial and reduction constants.  In device code each is made where
+ * it is used, by two scalar moves the compiler may not hoist: otherwise it
+ * lifts every coefficient to the kernel's entry, and in the shade kernels,
+ * which hold many live values, parks them in spill lanes and reads them
+ * back around each use.  The bits are those of the literal either way. */
+#if defined(__HIP_DEVICE_COMPILE__)
+template <unsigned long long B>
+__device__ static inline double ynf_kk()
+{
+    unsigned lo, hi;
+    __asm__ volatile("s_mov_b32 %0, %1" : "=s"(lo) : "i"((unsigned)(B & 0xffffffffull)));
+    __asm__ volatile("s_mov_b32 %0, %1" : "=s"(hi) : "i"((unsigned)(B >> 32)));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+#define YNF_K(d) (ynf_kk<__builtin_bit_cast(unsigned long long, (double)(d))>())
+#else
+#define YNF_K(d) (d)
+#endif
+
 YN_FM uint64_t ynf_bits(double x)
 {
     uint64_t u;
@@ -57,7 +76,7 @@ YN_FM double ynf_from_bits(uint64_t u)
  * 1.5 * 2^52 leaves no fraction bits) */
 YN_FM double ynf_rint(double x)
 {
-    const double big = 6755399441055744.0;
+    const double big = YNF_K(6755399441055744.0);
     double t = x + big;
     return t - big;
 }
@@ -66,26 +85,26 @@ YN_FM double ynf_rint(double x)
 YN_FM double ynf_sin_k(double r)
 {
     double z = r * r;
-    double p = -7.647163731819816e-13;            /* -1/15! */
-    p = p * z + 1.6059043836821613e-10;           /*  1/13! */
-    p = p * z + -2.505210838544172e-08;           /* -1/11! */
-    p = p * z + 2.7557319223985893e-06;           /*  1/9!  */
-    p = p * z + -0.0001984126984126984;           /* -1/7!  */
-    p = p * z + 0.008333333333333333;             /*  1/5!  */
-    p = p * z + -0.16666666666666666;             /* -1/3!  */
+    double p = YNF_K(-7.647163731819816e-13);            /* -1/15! */
+    p = p * z + YNF_K(1.6059043836821613e-10);           /*  1/13! */
+    p = p * z + YNF_K(-2.505210838544172e-08);           /* -1/11! */
+    p = p * z + YNF_K(2.7557319223985893e-06);           /*  1/9!  */
+    p = p * z + YNF_K(-0.0001984126984126984);           /* -1/7!  */
+    p = p * z + YNF_K(0.008333333333333333);             /*  1/5!  */
+    p = p * z + YNF_K(-0.16666666666666666);             /* -1/3!  */
     return r + (r * z) * p;
 }
 
 YN_FM double ynf_cos_k(double r)
 {
     double z = r * r;
-    double p = 4.779477332387385e-14;             /*  1/16! */
-    p = p * z + -1.1470745597729725e-11;          /* -1/14! */
-    p = p * z + 2.08767569878681e-09;             /*  1/12! */
-    p = p * z + -2.755731922398589e-07;           /* -1/10! */
-    p = p * z + 2.48015873015873e-05;             /*  1/8!  */
-    p = p * z + -0.001388888888888889;            /* -1/6!  */
-    p = p * z + 0.041666666666666664;             /*  1/4!  */
+    double p = YNF_K(4.779477332387385e-14);             /*  1/16! */
+    p = p * z + YNF_K(-1.1470745597729725e-11);          /* -1/14! */
+    p = p * z + YNF_K(2.08767569878681e-09);             /*  1/12! */
+    p = p * z + YNF_K(-2.755731922398589e-07);           /* -1/10! */
+    p = p * z + YNF_K(2.48015873015873e-05);             /*  1/8!  */
+    p = p * z + YNF_K(-0.001388888888888889);            /* -1/6!  */
+    p = p * z + YNF_K(0.041666666666666664);             /*  1/4!  */
     p = p * z + -0.5;                             /* -1/2!  */
     return 1.0 + z * p;
 }
@@ -93,8 +112,8 @@ YN_FM double ynf_cos_k(double r)
 /* x = k pi/2 + r; returns r, quadrant k mod 4 in *q */
 YN_FM double ynf_reduce(double x, int* q)
 {
-    double k = ynf_rint(x * YNF_TWO_OVER_PI);
-    double r = (x - k * YNF_PIO2_HI) - k * YNF_PIO2_LO;
+    double k = ynf_rint(x * YNF_K(YNF_TWO_OVER_PI));
+    double r = (x - k * YNF_K(YNF_PIO2_HI)) - k * YNF_K(YNF_PIO2_LO);
     *q = (int)((long long)k & 3);
     return r;
 }
@@ -124,44 +143,44 @@ YN_FM double ynf_log(double x)
     uint64_t b = ynf_bits(x);
     int e = (int)((b >> 52) & 0x7ff) - 1023;
     double m = ynf_from_bits((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull);   /* [1, 2) */
-    if (m > YNF_SQRT2) {
+    if (m > YNF_K(YNF_SQRT2)) {
         m = m * 0.5;
         e += 1;
     }
     double s = (m - 1.0) / (m + 1.0);
     double z = s * s;
-    double p = 0.047619047619047616;              /* 1/21 */
-    p = p * z + 0.05263157894736842;              /* 1/19 */
-    p = p * z + 0.058823529411764705;             /* 1/17 */
-    p = p * z + 0.06666666666666667;              /* 1/15 */
-    p = p * z + 0.07692307692307693;              /* 1/13 */
-    p = p * z + 0.09090909090909091;              /* 1/11 */
-    p = p * z + 0.1111111111111111;               /* 1/9  */
-    p = p * z + 0.14285714285714285;              /* 1/7  */
-    p = p * z + 0.2;                              /* 1/5  */
-    p = p * z + 0.3333333333333333;               /* 1/3  */
+    double p = YNF_K(0.047619047619047616);              /* 1/21 */
+    p = p * z + YNF_K(0.05263157894736842);              /* 1/19 */
+    p = p * z + YNF_K(0.058823529411764705);             /* 1/17 */
+    p = p * z + YNF_K(0.06666666666666667);              /* 1/15 */
+    p = p * z + YNF_K(0.07692307692307693);              /* 1/13 */
+    p = p * z + YNF_K(0.09090909090909091);              /* 1/11 */
+    p = p * z + YNF_K(0.1111111111111111);               /* 1/9  */
+    p = p * z + YNF_K(0.14285714285714285);              /* 1/7  */
+    p = p * z + YNF_K(0.2);                              /* 1/5  */
+    p = p * z + YNF_K(0.3333333333333333);               /* 1/3  */
     p = p * z + 1.0;
     double lm = 2.0 * (s * p);
     double de = (double)e;
-    return de * YNF_LN2_HI + (lm + de * YNF_LN2_LO);
+    return de * YNF_K(YNF_LN2_HI) + (lm + de * YNF_K(YNF_LN2_LO));
 }
 
 /* e^z for |z| <= 110 */
 YN_FM double ynf_exp(double z)
 {
-    double k = ynf_rint(z * YNF_INV_LN2);
-    double r = (z - k * YNF_LN2_HI) - k * YNF_LN2_LO;
-    double p = 1.6059043836821613e-10;            /* 1/13! */
-    p = p * r + 2.08767569878681e-09;             /* 1/12! */
-    p = p * r + 2.505210838544172e-08;            /* 1/11! */
-    p = p * r + 2.755731922398589e-07;            /* 1/10! */
-    p = p * r + 2.7557319223985893e-06;           /* 1/9!  */
-    p = p * r + 2.48015873015873e-05;             /* 1/8!  */
-    p = p * r + 0.0001984126984126984;            /* 1/7!  */
-    p = p * r + 0.001388888888888889;             /* 1/6!  */
-    p = p * r + 0.008333333333333333;             /* 1/5!  */
-    p = p * r + 0.041666666666666664;             /* 1/4!  */
-    p = p * r + 0.16666666666666666;              /* 1/3!  */
+    double k = ynf_rint(z * YNF_K(YNF_INV_LN2));
+    double r = (z - k * YNF_K(YNF_LN2_HI)) - k * YNF_K(YNF_LN2_LO);
+    double p = YNF_K(1.6059043836821613e-10);            /* 1/13! */
+    p = p * r + YNF_K(2.08767569878681e-09);             /* 1/12! */
+    p = p * r + YNF_K(2.505210838544172e-08);            /* 1/11! */
+    p = p * r + YNF_K(2.755731922398589e-07);            /* 1/10! */
+    p = p * r + YNF_K(2.7557319223985893e-06);           /* 1/9!  */
+    p = p * r + YNF_K(2.48015873015873e-05);             /* 1/8!  */
+    p = p * r + YNF_K(0.0001984126984126984);            /* 1/7!  */
+    p = p * r + YNF_K(0.001388888888888889);             /* 1/6!  */
+    p = p * r + YNF_K(0.008333333333333333);             /* 1/5!  */
+    p = p * r + YNF_K(0.041666666666666664);             /* 1/4!  */
+    p = p * r + YNF_K(0.16666666666666666);              /* 1/3!  */
     p = p * r + 0.5;                              /* 1/2!  */
     p = p * r + 1.0;
     p = p * r + 1.0;
