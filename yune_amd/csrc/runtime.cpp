@@ -235,7 +235,7 @@ int alloc_wavefront(yn_ctx* c, int64_t n)
     r |= A(&P.job); r |= A(&P.hit); r |= A(&P.occ);
     for (int k = 0; k < 2; ++k) {       // path records beside queues 0 and 1
         PathRecs& R = P.R[k];
-        int rr = dalloc(c, &R.base, (size_t)n * PathRecs::FIELDS);
+        int rr = dalloc(c, &R.base, (size_t)PathRecs::alloc_entries(n));
         if (rr == 0) c->allocs.push_back(R.base);
         r |= rr;
         R.cap = n;

@@ -59,10 +59,15 @@ struct RayQueue {
 // so records move with their rays in queue order and every access is
 // coalesced.
 struct PathRecs {
-    // Thirteen 16-byte fields per record, each field an array of `cap`
-    // entries in one allocation: field f of record r at base[f * cap + r].
-    // One base pointer (not thirteen) keeps the kernels' argument registers
-    // few.
+    // Thirteen 16-byte fields per record in one allocation, in tiles of
+    // kTile records: tile t holds field 0 of its records, then field 1, ...
+    // (field f of record r at base[((r / kTile) * FIELDS + f) * kTile +
+    // r % kTile]).  A wave's 64 consecutive records still read each field
+    // as two whole 512-byte runs, and all thirteen fields of a record lie
+    // within +-3 KB of its field 6, so one address per record and immediate
+    // offsets reach them (a field-major layout needs one 64-bit base per
+    // field, which the shade kernels could only keep in spill lanes).
+    static constexpr int kTile = 32;
     float4* base;
     int64_t cap;                // records (paths in flight at most)
     enum Field {
@@ -81,7 +86,12 @@ struct PathRecs {
         MIS,                    // NEE: MIS BRDF direction, w = BRDF pdf
         FIELDS
     };
-    __host__ __device__ float4* f(int field, int64_t r) const { return base + (size_t)field * (size_t)cap + (size_t)r; }
+    __host__ __device__ float4* f(int field, int64_t r) const
+    {
+        return base + ((size_t)(r / kTile) * FIELDS + (size_t)field) * kTile + (size_t)(r % kTile);
+    }
+    // float4 entries to allocate for n records (whole tiles)
+    static constexpr int64_t alloc_entries(int64_t n) { return (n + kTile - 1) / kTile * kTile * FIELDS; }
 };
 
 // Path states, structure of arrays.
