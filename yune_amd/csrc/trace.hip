@@ -28,36 +28,11 @@
 // chunks of rays from the queue with one atomic per chunk.
 #include "yn_device.h"
 #include "yn_internal_dev.h"
+#include "yn_slab.h"
 
 namespace yune {
 
 namespace {
-
-// One slab of rayAabbIntersection (udpt.cl:397-418).  min_diff can only be
-// NaN when dir_inv is infinite (a zero or denormal direction component) and
-// the plane passes through the origin, where the reference skips the slab.
-__device__ __forceinline__ void slab(float lo, float hi, float o, float inv, float& tmin, float& tmax)
-{
-    float a = (lo - o) * inv;           // min_diff (udpt.cl:397)
-    float b = (hi - o) * inv;           // max_diff (udpt.cl:398)
-    if (!(a != a)) {                    // !isnan(min_diff)
-        tmin = fmaxf(fminf(a, b), tmin);
-        tmax = fminf(fmaxf(a, b), tmax);
-    }
-}
-
-// rayAabbIntersection (udpt.cl:392-431) + entry distance for culling.
-__device__ __forceinline__ bool box_hit(float lox, float loy, float loz, float hix, float hiy, float hiz,
-                                        float ox, float oy, float oz, float ix, float iy, float iz,
-                                        float& enter)
-{
-    float tmin = -INFINITY, tmax = INFINITY;
-    slab(lox, hix, ox, ix, tmin, tmax);
-    slab(loy, hiy, oy, iy, tmin, tmax);
-    slab(loz, hiz, oz, iz, tmin, tmax);
-    enter = fmaxf(tmin, 0.0f);
-    return tmax > enter;
-}
 
 struct RayState {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
@@ -69,46 +44,6 @@ struct RayState {
     float u, v;             //   the slot, with id, u and v recomputed at the end if wanted
 };
 
-constexpr float kWiden = 1.0f + 0x1p-19f;   // relative widening of the FMA slab form
-
-// The cull distance for a best hit t: the margin 1e-4 the traversal always
-// had, widened as the FMA slab form's exit distance is (below), so that a
-// node the reference's arithmetic would enter is never culled by it.
-__device__ __forceinline__ float cull_of(float t) { return t * 1.0001f * kWiden; }
-
-// The slab test with o * dir_inv precomputed per ray: each slab distance is
-// one FMA, plane * dir_inv - o * dir_inv, where the reference takes a
-// difference and a product (u = 2^-24 below).  Rounding o * dir_inv costs
-// up to u|o * dir_inv| in absolute terms; the offsets subtracted are moved
-// by 8u|o * dir_inv| (2^-21, exact), towards smaller distances for the
-// planes the ray enters through and larger ones for those it leaves
-// through (which of lo and hi that is follows the sign of dir_inv), which
-// covers that error and the rounding of the move.  What remains is
-// relative: each entry distance is at most 4u of its own size above the
-// reference's, each exit distance at most 4u below; min and max are
-// monotone, so the same holds for the box's entry and exit.  The exit
-// distance is therefore widened by kWiden = 1 + 32u before the comparison,
-// with >= where the reference has >, and cull distances likewise: every box
-// the reference's arithmetic enters is entered here.  The triangle tests
-// are exact and the closest hit is the least t over the triangles tested,
-// so the few boxes visited beyond the reference's change a result only
-// where its arithmetic drops a box that a triangle hit touches to within
-// rounding (a hit on the box's boundary), as a tie in t would.  Used for
-// rays with |dir_inv| and |o| at most 2^40 in scenes whose box coordinates
-// are finite and at most 2^60, where nothing can overflow; other rays take
-// box_hit.
-__device__ __forceinline__ bool box_hit_fma(float lox, float loy, float loz, float hix, float hiy, float hiz,
-                                            const RayState& r, float& enter)
-{
-    const float ax = fmaf(lox, r.ix, -r.lx), bx = fmaf(hix, r.ix, -r.hx);
-    const float ay = fmaf(loy, r.iy, -r.ly), by = fmaf(hiy, r.iy, -r.hy);
-    const float az = fmaf(loz, r.iz, -r.lz), bz = fmaf(hiz, r.iz, -r.hz);
-    const float tmin = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
-    const float tmax = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
-    enter = fmaxf(tmin, 0.0f);
-    return tmax * kWiden >= enter;
-}
-
 // A child box of an interior node: the reference's slab arithmetic (EXACT)
 // or the widened FMA form.
 template <bool EXACT>
@@ -116,7 +51,8 @@ __device__ __forceinline__ bool child_hit(float lox, float loy, float loz, float
                                           const RayState& r, float& enter)
 {
     return EXACT ? box_hit(lox, loy, loz, hix, hiy, hiz, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, enter)
-                 : box_hit_fma(lox, loy, loz, hix, hiy, hiz, r, enter);
+                 : box_hit_fma(lox, loy, loz, hix, hiy, hiz, r.ix, r.iy, r.iz, r.lx, r.ly, r.lz, r.hx, r.hy, r.hz,
+                               enter);
 }
 
 // A ray's traversal state from its queue entry.  Returns whether the FMA
@@ -126,21 +62,13 @@ __device__ __forceinline__ bool start_ray(RayState& r, const float4& ot, const f
     r.ox = ot.x; r.oy = ot.y; r.oz = ot.z;
     r.dx = dp.x; r.dy = dp.y; r.dz = dp.z;
     r.ix = 1 / r.dx; r.iy = 1 / r.dy; r.iz = 1 / r.dz;   // dir_inv (udpt.cl:395)
-    auto offsets = [](float o, float inv, float& l, float& h) {
-        const float oi = o * inv, e = fabsf(oi) * 0x1p-21f;
-        // entering through lo when dir_inv > 0, through hi when < 0
-        l = inv > 0.0f ? oi + e : oi - e;
-        h = inv > 0.0f ? oi - e : oi + e;
-    };
-    offsets(r.ox, r.ix, r.lx, r.hx);
-    offsets(r.oy, r.iy, r.ly, r.hy);
-    offsets(r.oz, r.iz, r.lz, r.hz);
+    fma_offsets(r.ox, r.ix, r.lx, r.hx);
+    fma_offsets(r.oy, r.iy, r.ly, r.hy);
+    fma_offsets(r.oz, r.iz, r.lz, r.hz);
     r.best = ot.w;
     r.cull = cull_of(ot.w);
     r.tri = -1; r.u = 0.0f; r.v = 0.0f;
-    const float lim = 0x1p40f;
-    return S.fma_slab && fabsf(r.ix) <= lim && fabsf(r.iy) <= lim && fabsf(r.iz) <= lim && fabsf(r.ox) <= lim &&
-           fabsf(r.oy) <= lim && fabsf(r.oz) <= lim;
+    return S.fma_slab && fma_form_ok(r.ox, r.oy, r.oz, r.ix, r.iy, r.iz);
 }
 
 // A load from a scene array as a global-memory load.  Through a generic
