@@ -51,31 +51,40 @@ struct Vtx {
     int tri;
 };
 
-__device__ __forceinline__ size_t vslot(const BdptSoA& B, int side, int k, int p)
+using BF = BdptRecs;
+
+// the position field of vertex k of a side; normal, direction and
+// contribution follow it
+__device__ __forceinline__ float4* vfield(const BdptSoA& B, int side, int k, int p)
 {
-    return (size_t)(side * B.K + k) * (size_t)B.cap + (size_t)p;
+    return B.R.f(BF::VTX + 4 * (side * B.K + k), p);
 }
 
 __device__ __forceinline__ Vtx load_v(const BdptSoA& B, int side, int k, int p)
 {
-    size_t i = vslot(B, side, k, p);
-    float4 q = B.vp[i];
+    const float4* f = vfield(B, side, k, p);
+    float4 q = f[0];
     Vtx v;
     v.p = mk(q.x, q.y, q.z, 1.0f);
     v.tri = __float_as_int(q.w);
-    v.n = ld(B.vn[i]);
-    v.d = ld(B.vd[i]);
-    v.c = ld(B.vc[i]);
+    v.n = ld(f[BF::kTile]);
+    v.d = ld(f[2 * BF::kTile]);
+    v.c = ld(f[3 * BF::kTile]);
     return v;
 }
 
 __device__ __forceinline__ void store_v(const BdptSoA& B, int side, int k, int p, const Vtx& v)
 {
-    size_t i = vslot(B, side, k, p);
-    B.vp[i] = make_float4(v.p.x, v.p.y, v.p.z, __int_as_float(v.tri));
-    B.vn[i] = st4(v.n);
-    B.vd[i] = st4(v.d);
-    B.vc[i] = st4(v.c);
+    float4* f = vfield(B, side, k, p);
+    f[0] = make_float4(v.p.x, v.p.y, v.p.z, __int_as_float(v.tri));
+    f[BF::kTile] = st4(v.n);
+    f[2 * BF::kTile] = st4(v.d);
+    f[3 * BF::kTile] = st4(v.c);
+}
+
+__device__ __forceinline__ int4 ld_i4(const BdptSoA& B, int field, int p)
+{
+    return *reinterpret_cast<const int4*>(B.R.f(field, p));
 }
 
 __device__ __forceinline__ void write_sample(const DevJob& J, int slot, int pix, f4 c)
@@ -119,17 +128,19 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
     const bool fix = RP.fix_mis != 0;
     const int KB = RP.bdpt_bounces;
 
-    int st = P.state[p];
+    const int4 i0 = ld_i4(B, BF::I0, p), i1 = ld_i4(B, BF::I1, p), i2 = ld_i4(B, BF::I2, p);
+    int st = i0.y;
     const int code = st & B_MASK;
-    uint32_t seed = P.seed[p];
-    const int pix = P.pix[p];
-    const int slot = P.slot[p];
+    uint32_t seed = (uint32_t)i0.x;
+    const int pix = i0.w;
+    const int slot = i1.x;
     const bool cap_on = CAP && cap.prim_tri != nullptr && slot == 0;
-    int k = P.bounce[p];
-    int lp_len = B.lp_len[p], ep_len = B.ep_len[p];
-    float w_eye = B.w_eye[p];
-    f4 color = ld(P.direct[p]);
-    uint64_t conn = B.conn[p];
+    int k = i0.z;
+    int lp_len = i1.y, ep_len = i1.z;
+    float w_eye = __int_as_float(i1.w);
+    f4 color = ld(*B.R.f(BF::DIRECT, p));
+    uint64_t conn = (uint64_t)(uint32_t)i2.x | ((uint64_t)(uint32_t)i2.y << 32);
+    float nee_prob = __int_as_float(i2.z);     // NEE lobe probability, carried between rounds
 
     enum Step { T_LIGHT_NEXT, T_EYE_START, T_EYE_NEXT, T_ROUND_START, T_NEE_SHADOW, T_NEE_MIS, T_CONNECT,
                 T_FINISH, T_FINISH_CONST };
@@ -149,7 +160,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
                 cap.prim_light[pix] = (htri < 0 && (st & BF_LCAND)) ? 0 : -1;
                 cap.prim_t[pix] = h.x;
             }
-            f4 d = ld(B.eye_d[p]);
+            f4 d = ld(*B.R.f(BF::EYE_D, p));
             if (htri < 0) {
                 if (st & BF_LCAND)
                     const_color = dot(d, ld(L.normal)) < 0 ? mk(1, 1, 1, 1) : mk(0.1f, 0.1f, 0.1f, 1);
@@ -157,7 +168,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
                     const_color = mk(RP.background[0], RP.background[1], RP.background[2], RP.background[3]);
                 step = T_FINISH_CONST;
             } else {
-                f4 o = ld(B.eye_o[p]);
+                f4 o = ld(*B.R.f(BF::EYE_O, p));
                 cur.p = o + d * h.x;
                 cur.n = shade_normal(S.tri_shade, htri, h.z, h.w);
                 cur.d = d;
@@ -171,8 +182,8 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
             step = side == 0 ? T_EYE_START : T_ROUND_START;
         } else {
             Vtx pv = load_v(B, side, k - 1, p);
-            f4 d = ld(P.nd[p]);
-            float4 bnc = P.bnc[p];
+            f4 d = ld(*B.R.f(BF::ND, p));
+            float4 bnc = *B.R.f(BF::BNC, p);
             float pdf = bnc.x, prob = bnc.y;
             bool g = (st & BF_BGLOSSY) != 0;
             f4 o = pv.p + d * kEps;
@@ -229,8 +240,8 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
         st &= ~(BF_LCAND | BF_BGLOSSY);
         if (quad_hit(L, o, d, INFINITY, tq)) { tmax = tq; st |= BF_LCAND; }
         if (g) st |= BF_BGLOSSY;
-        P.nd[p] = st4(d);
-        P.bnc[p] = make_float4(pdf, prob, 0.0f, 0.0f);
+        *B.R.f(BF::ND, p) = st4(d);
+        *B.R.f(BF::BNC, p) = make_float4(pdf, prob, 0.0f, 0.0f);
         st = (st & ~B_MASK) | (light ? B_LIGHT : B_EYE);
         k = k + 1;
         out.ext = true;
@@ -238,7 +249,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
         goto emitted;
     } while (0);
     if (step == T_EYE_START) do {          // the camera ray (createEyePath :517)
-        f4 o = ld(B.eye_o[p]), d = ld(B.eye_d[p]);
+        f4 o = ld(*B.R.f(BF::EYE_O, p)), d = ld(*B.R.f(BF::EYE_D, p));
         float tq, tmax = INFINITY;
         st &= ~(BF_LCAND | BF_BGLOSSY);
         if (quad_hit(L, o, d, INFINITY, tq)) { tmax = tq; st |= BF_LCAND; }
@@ -274,7 +285,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
                 }
             }
             if (nee_done) {
-                P.ls[p] = st4(m.ke);
+                *B.R.f(BF::LS, p) = st4(m.ke);
                 st |= BF_NEE_DONE;
             } else {
                 f4 so = e.p + wi * kEps;
@@ -282,8 +293,8 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
                     cap.shadow_o[pix] = make_float4(so.x, so.y, so.z, len);
                     cap.shadow_d[pix] = make_float4(wi.x, wi.y, wi.z, 1.0f);
                 }
-                P.wi[p] = make_float4(wi.x, wi.y, wi.z, lpdf);
-                P.nee_prob[p] = nprob;
+                *B.R.f(BF::WI, p) = make_float4(wi.x, wi.y, wi.z, lpdf);
+                nee_prob = nprob;
                 float tq;
                 if (quad_hit(L, so, wi, len, tq)) st |= BF_NEE_Q;
                 else { conn |= 1ull; set_ray(out, so, wi, len, p << B.shift); }
@@ -305,13 +316,13 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
         if (step == T_NEE_SHADOW) do {         // udpt.cl:557-597 (bdpt variant of the BRDF pieces)
             Vtx e = load_v(B, 1, k, p);
             Mat m = load_mat(S.mats, S.tri_mat[e.tri]);
-            if (st & BF_NEE_DONE) { nee_res = ld(P.ls[p]); step = T_CONNECT; break; }
+            if (st & BF_NEE_DONE) { nee_res = ld(*B.R.f(BF::LS, p)); step = T_CONNECT; break; }
             int occ = (st & BF_NEE_Q) ? 1 : B.occ[((size_t)p << B.shift)];
             if (cap_on && k == 0 && !(st & BF_CAPSH)) { cap.shadow_occ[pix] = occ; st |= BF_CAPSH; }
-            float4 w4 = P.wi[p];
+            float4 w4 = *B.R.f(BF::WI, p);
             f4 wi = mk(w4.x, w4.y, w4.z, 0.0f);
             float lpdf = w4.w;
-            float nprob = P.nee_prob[p];
+            float nprob = nee_prob;
             f4 em = m.ke;
             f4 w_o = -e.d;
             bool glossy = fix ? ((st & BF_NGLOSSY) != 0) : false;
@@ -334,9 +345,9 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
             float tq;
             if (!quad_hit(L, bo, bd, INFINITY, tq)) { nee_res = ls + em; step = T_CONNECT; break; }
             st = glossy ? (st | BF_NGLOSSY) : (st & ~BF_NGLOSSY);
-            P.ls[p] = st4(ls);
-            P.mis[p] = make_float4(bd.x, bd.y, bd.z, bpdf);
-            P.nee_prob[p] = nprob;
+            *B.R.f(BF::LS, p) = st4(ls);
+            *B.R.f(BF::MIS, p) = make_float4(bd.x, bd.y, bd.z, bpdf);
+            nee_prob = nprob;
             st = (st & ~B_MASK) | B_MIS;
             conn |= 1ull;
             out.emit = 1ull;
@@ -348,14 +359,14 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
             Vtx e = load_v(B, 1, k, p);
             Mat m = load_mat(S.mats, S.tri_mat[e.tri]);
             f4 em = m.ke;
-            f4 ls = ld(P.ls[p]);
+            f4 ls = ld(*B.R.f(BF::LS, p));
             int occ = B.occ[((size_t)p << B.shift)];
             if (occ) { nee_res = ls + em; step = T_CONNECT; break; }
-            float4 mb = P.mis[p];
+            float4 mb = *B.R.f(BF::MIS, p);
             f4 bd = mk(mb.x, mb.y, mb.z, 0.0f);
             float bpdf = mb.w;
-            float lpdf = P.wi[p].w;
-            float nprob = P.nee_prob[p];
+            float lpdf = B.R.f(BF::WI, p)->w;
+            float nprob = nee_prob;
             bool glossy = (st & BF_NGLOSSY) != 0;
             float mw = power_heuristic(bpdf, lpdf, bpdf);
             f4 bs = (eval_brdf(m, bd, -e.d, e.p, e.n, glossy, nprob, false, true) * ld(L.ke)) * fmaxf(dot(bd, e.n), 0.0f);
@@ -407,14 +418,11 @@ emitted:
         write_sample(J, slot, pix, mk(NAN, NAN, NAN, NAN));
         ended = true;
     }
-    P.state[p] = ended ? B_DONE : st;
-    P.seed[p] = seed;
-    P.bounce[p] = k;
-    B.lp_len[p] = lp_len;
-    B.ep_len[p] = ep_len;
-    B.w_eye[p] = w_eye;
-    B.conn[p] = conn;
-    P.direct[p] = st4(color);
+    *reinterpret_cast<int4*>(B.R.f(BF::I0, p)) = make_int4((int)seed, ended ? B_DONE : st, k, pix);
+    *reinterpret_cast<int4*>(B.R.f(BF::I1, p)) = make_int4(slot, lp_len, ep_len, __float_as_int(w_eye));
+    *reinterpret_cast<int4*>(B.R.f(BF::I2, p)) =
+        make_int4((int)(uint32_t)conn, (int)(uint32_t)(conn >> 32), __float_as_int(nee_prob), 0);
+    *B.R.f(BF::DIRECT, p) = st4(color);
     return out;
 }
 
@@ -501,8 +509,8 @@ __global__ __launch_bounds__(256) void k_generate_bdpt(GenArgs a, DevCapture cap
     float r2 = next_rand(seed);
     f4 eo, ed;
     create_ray((float)x + r1, (float)y + r2, J.W, J.H, J.cam, eo, ed);
-    B.eye_o[p] = st4(eo);
-    B.eye_d[p] = st4(ed);
+    *B.R.f(BF::EYE_O, p) = st4(eo);
+    *B.R.f(BF::EYE_D, p) = st4(ed);
     // light vertex 0: r1 by a double division (bdpt.cl:439), r1/r2 swapped on the edges
     seed = xor_shift(seed);
     float lr1 = (float)((double)seed / (double)4294967295u);
@@ -526,27 +534,21 @@ __global__ __launch_bounds__(256) void k_generate_bdpt(GenArgs a, DevCapture cap
     if (pdf > 0.0f) {
         o = v0.p + d * kEps;
         st = B_LIGHT;
-        a.P.nd[p] = st4(d);
-        a.P.bnc[p] = make_float4(pdf, 0.0f, 0.0f, 0.0f);
-        a.P.bounce[p] = 1;
+        *B.R.f(BF::ND, p) = st4(d);
+        *B.R.f(BF::BNC, p) = make_float4(pdf, 0.0f, 0.0f, 0.0f);
     } else {                                // no light subpath beyond vertex 0: the camera ray
         o = eo;
         d = ed;
         st = B_EYE;
-        a.P.bounce[p] = 0;
     }
     float tq, tmax = INFINITY;
     if (quad_hit(L, o, d, INFINITY, tq)) { tmax = tq; st |= BF_LCAND; }
-    a.P.seed[p] = seed;
-    a.P.pix[p] = y * J.W + x;
     a.P.job[p] = g.job;
-    a.P.slot[p] = g.slot;
-    a.P.state[p] = st;
-    a.P.direct[p] = make_float4(0, 0, 0, 1);
-    B.lp_len[p] = 1;
-    B.ep_len[p] = 0;
-    B.w_eye[p] = 1.0f;
-    B.conn[p] = 0;
+    // bounce: 1 when the first light ray leaves vertex 0, else 0 (the camera ray)
+    *reinterpret_cast<int4*>(B.R.f(BF::I0, p)) = make_int4((int)seed, st, pdf > 0.0f ? 1 : 0, y * J.W + x);
+    *reinterpret_cast<int4*>(B.R.f(BF::I1, p)) = make_int4(g.slot, 1, 0, __float_as_int(1.0f));
+    *reinterpret_cast<int4*>(B.R.f(BF::I2, p)) = make_int4(0, 0, 0, 0);
+    *B.R.f(BF::DIRECT, p) = make_float4(0, 0, 0, 1);
     a.ext_out.o_t[p] = make_float4(o.x, o.y, o.z, tmax);
     a.ext_out.d_p[p] = make_float4(d.x, d.y, d.z, __int_as_float(p));
     a.list_out[p] = (uint32_t)p;

@@ -267,10 +267,6 @@ void free_bdpt(yn_ctx* c)
     c->B = BdptSoA{};
     c->ShB[0] = c->ShB[1] = RayQueue{};
     c->lst[0] = c->lst[1] = nullptr;
-    PathSoA& P = c->P;
-    P.seed = nullptr; P.pix = nullptr; P.slot = nullptr; P.state = nullptr; P.bounce = nullptr;
-    P.direct = nullptr; P.nd = nullptr; P.bnc = nullptr; P.wi = nullptr; P.ls = nullptr; P.mis = nullptr;
-    P.nee_prob = nullptr;
     c->cap_bdpt = 0;
 }
 
@@ -286,10 +282,10 @@ int ensure_bdpt(yn_ctx* c, int bounces)
     int shift = 0;
     while ((1 << shift) < K) ++shift;
     const int64_t R = (int64_t)1 << shift;
-    // vertices, shadow results and queues, path lists, subpath bookkeeping
-    // (lengths, eye weight, camera ray, the 64-bit connection mask), and the
-    // per-path shading fields of PathSoA used by this mode
-    const int64_t per_path = 2 * K * 64 + R * 4 + 2 * R * 32 + 2 * 4 + 3 * 4 + 2 * 16 + 8 + 6 * 4 + 6 * 16;
+    // the per-path records (fixed fields and 2K vertices of 64 bytes), shadow
+    // results and queues, path lists
+    const int fields = BdptRecs::VTX + 8 * K;
+    const int64_t per_path = (int64_t)fields * 16 + R * 4 + 2 * R * 32 + 2 * 4;
     int64_t cap = std::min<int64_t>(c->cap_paths, (int64_t)(kBdptBudget / (size_t)per_path));
     cap = std::min<int64_t>(cap, ((int64_t)1 << (31 - shift)) - 1);
     if (cap < 1) return fail(c, YN_ERR_NO_MEMORY, "no room for bidirectional paths");
@@ -299,20 +295,15 @@ int ensure_bdpt(yn_ctx* c, int bounces)
         return r;
     };
     BdptSoA& B = c->B;
-    const size_t n = (size_t)cap, nv = (size_t)(2 * K) * n, ns = n << shift;
+    const size_t n = (size_t)cap, ns = n << shift;
     int r = 0;
-    r |= A(&B.eye_o, n); r |= A(&B.eye_d, n); r |= A(&B.lp_len, n); r |= A(&B.ep_len, n);
-    r |= A(&B.w_eye, n); r |= A(&B.conn, n);
-    r |= A(&B.vp, nv); r |= A(&B.vn, nv); r |= A(&B.vd, nv); r |= A(&B.vc, nv);
+    r |= A(&B.R.base, (size_t)BdptRecs::alloc_entries(cap, fields));
+    B.R.fields = fields;
     r |= A(&B.occ, ns);
     for (int k = 0; k < 2; ++k) {
         r |= A(&c->ShB[k].o_t, ns); r |= A(&c->ShB[k].d_p, ns);
         r |= A(&c->lst[k], n);
     }
-    PathSoA& P = c->P;
-    r |= A(&P.seed, n); r |= A(&P.pix, n); r |= A(&P.slot, n); r |= A(&P.state, n); r |= A(&P.bounce, n);
-    r |= A(&P.direct, n); r |= A(&P.nd, n); r |= A(&P.bnc, n); r |= A(&P.wi, n); r |= A(&P.ls, n);
-    r |= A(&P.mis, n); r |= A(&P.nee_prob, n);
     if (r) { free_bdpt(c); return r < 0 ? r : YN_ERR_HIP; }
     B.K = K;
     B.shift = shift;

@@ -99,36 +99,43 @@ struct PathSoA {
     int32_t* job;               // bidirectional: job slot per path (the trace kernels' scene lookup)
     float4* hit;                // closest-hit result: t, tri bits, u, v (per path / queue entry)
     int32_t* occ;               // any-hit result (per shadow-queue entry)
-    // bidirectional mode only (allocated with its state, one entry per path):
-    uint32_t* seed;
-    int32_t* pix;               // pixel index y*W + x
-    int32_t* slot;              // frame slot within the job's sample buffer
-    int32_t* state;             // state | flags
-    int32_t* bounce;            // vertex index of the subpath being built
-    float4* direct;             // colour accumulated so far
-    float4* nd;                 // pending bounce direction
-    float4* bnc;                // pending bounce: pdf, brdf_prob, -, -
-    float4* wi;                 // NEE: normalised w_i, w = light pdf
-    float4* ls;                 // NEE: light sample (MIS-weighted)
-    float4* mis;                // NEE: MIS brdf dir xyz, w = brdf pdf
-    float* nee_prob;            // NEE: lobe probability of the light sample
     PathRecs R[2];              // unidirectional: records beside queues 0 and 1
 };
 
-// Bidirectional paths (bdpt.cl): both subpaths' vertices plus the
-// per-path bookkeeping of the connection rounds.  Vertex arrays are
-// [side * K + k][cap] (side 0: light subpath, 1: eye subpath).
+// Bidirectional paths (bdpt.cl): the per-path state, both subpaths'
+// vertices and the bookkeeping of the connection rounds, in one allocation
+// of 16-byte fields in tiles of 32 paths, as PathRecs (field f of path p at
+// base[((p / 32) * fields + f) * 32 + p % 32]): a path's fields lie within
+// a few KB of each other, so the shade keeps few addresses and no base
+// pointer per field.
+struct BdptRecs {
+    static constexpr int kTile = 32;
+    enum Field {
+        I0,                     // int4 {seed, state | flags, bounce (vertex being built), pixel y*W + x}
+        I1,                     // int4 {frame slot, light subpath length, eye subpath length, eye_path_weight bits}
+        I2,                     // int4 {connection slots of the round (64 bits), NEE lobe probability bits, -}
+        DIRECT,                 // colour accumulated so far
+        ND,                     // pending bounce direction
+        BNC,                    // pending bounce: pdf, brdf_prob, -, -
+        WI,                     // NEE: normalised w_i, w = light pdf
+        LS,                     // NEE: light sample (MIS-weighted)
+        MIS,                    // NEE: MIS brdf dir xyz, w = brdf pdf
+        EYE_O,                  // camera ray, kept until the eye subpath starts
+        EYE_D,
+        VTX                     // vertex v = side * K + k (side 0: light, 1: eye) at VTX + 4 v:
+    };                          //   position (w = triangle id bits, -1: on the light), normal,
+                                //   direction of the ray that found it, contribution
+    float4* base;
+    int32_t fields;             // VTX + 8 K
+    __host__ __device__ float4* f(int field, int64_t p) const
+    {
+        return base + ((size_t)(p / kTile) * (size_t)fields + (size_t)field) * kTile + (size_t)(p % kTile);
+    }
+    static int64_t alloc_entries(int64_t n, int fields) { return (n + kTile - 1) / kTile * kTile * fields; }
+};
+
 struct BdptSoA {
-    float4* eye_o;              // camera ray, kept until the eye subpath starts
-    float4* eye_d;
-    int32_t* lp_len;            // light subpath length (vertex 0 lies on the quad)
-    int32_t* ep_len;            // eye subpath length
-    float* w_eye;               // eye_path_weight (bdpt.cl:568)
-    uint64_t* conn;             // shadow-ray slots issued in the current round (bit j: slot j)
-    float4* vp;                 // vertex position, w = triangle id bits (-1: on the light)
-    float4* vn;                 // shading normal
-    float4* vd;                 // direction of the ray that found the vertex
-    float4* vc;                 // contribution (PathInfo.contrib)
+    BdptRecs R;
     int32_t* occ;               // [cap << shift] any-hit results of a round
     int32_t K;                  // vertices kept per subpath
     int32_t shift;              // log2 of shadow-ray slots per path and round
