@@ -39,7 +39,7 @@ rocprofv3 counter summaries committed under profiles/ (taken with this
 file's own workloads and launch options); they are scaled by this run's
 units per launch.
 
---gpus above 1 is refused: this build runs on one GPU.
+--gpus above 1 is refused: the benchmark runs on one GPU.
 """
 from __future__ import annotations
 

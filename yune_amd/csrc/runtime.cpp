@@ -1350,7 +1350,7 @@ int yn_create(const yn_config* cfg, yn_ctx** out)
     if (hipGetDeviceProperties(&prop, k.device) != hipSuccess)
         return fail(nullptr, YN_ERR_NO_DEVICE, "hipGetDeviceProperties failed");
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-        return fail(nullptr, YN_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+        return fail(nullptr, YN_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", the library is built for gfx950");
     yn_ctx* c = new yn_ctx();
     c->cfg = k;
     c->device = k.device;
