@@ -1260,7 +1260,10 @@ int build_scene(yn_ctx* c, int sid, int width)
     D.brute = L.brute;
     D.n_slots = (int32_t)L.slot_tri.size();
     D.n_hoist = L.n_hoist;
-    D.fma_slab = fma_slab_ok(L) ? 1 : 0;
+    // YUNE_SLAB_REFERENCE=1 keeps the scene's rays on the reference's slab
+    // arithmetic, for comparing the two forms on one scene
+    const char* ref_slab = std::getenv("YUNE_SLAB_REFERENCE");
+    D.fma_slab = fma_slab_ok(L) && !(ref_slab && ref_slab[0] == '1') ? 1 : 0;
     HIPCK(c, hipMemcpy(c->d_scenes + sid, &D, sizeof(DevScene), hipMemcpyHostToDevice));
     S.info.n_tri = (int32_t)S.h_tris.size();
     S.info.n_ref_nodes = (int32_t)S.h_nodes.size();
