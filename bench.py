@@ -254,7 +254,9 @@ def serve_run(Y, np, serve, rate, duration, n_jobs, plain_jobs, frame_jobs, serv
         "workload": ("Poisson arrivals at %.1f jobs/s for %.1f s" % (rate, duration) if serve else
                      "%d jobs submitted together" % len(jobs)) +
                     ", 8 icosphere scenes (20k-2M tris), random cameras, 128^2-1024^2, 16-256 spp, depth 8, "
-                    "NEE+MIS, 64 GB image budget, %s" % ("continuous batching" if serve_mode == 0 else "rounds"),
+                    "NEE+MIS, 64 GB image budget, %s%s" % ("continuous batching" if serve_mode == 0 else "rounds",
+                                                          ", bounce loop replayed from a HIP graph"
+                                                          if launch.get("graph") else ""),
         "jobs": len(jobs), "scene_tris": [len(s.tris) for s in scenes],
         "total_megasamples": round(samples / 1e6, 1),
         "launch": launch, "autotuned": tuned is not None,
@@ -263,8 +265,10 @@ def serve_run(Y, np, serve, rate, duration, n_jobs, plain_jobs, frame_jobs, serv
         "makespan_s": round(makespan, 3),
         "served_mrays_per_s": round((sst.rays_ext + sst.rays_shadow) / makespan / 1e6, 1),
         "served_msamples_per_s": round(samples / makespan / 1e6, 1),
-        "served_kernels_ms": {"trace_ext": round(sst.ms_trace_ext, 1), "trace_shadow": round(sst.ms_trace_shadow, 1),
-                              "shade": round(sst.ms_shade, 1), "host_between_groups": round(sst.ms_host, 1)},
+        # kernel times are taken launch by launch; a replayed graph has none
+        "served_kernels_ms": ({"trace_ext": round(sst.ms_trace_ext, 1), "trace_shadow": round(sst.ms_trace_shadow, 1),
+                               "shade": round(sst.ms_shade, 1)} if not launch.get("graph") else None),
+        "host_between_groups_ms": round(sst.ms_host, 1),
         "served_loop": {"groups": sst.groups, "refills": sst.refills, "plan_stalls": sst.plan_stalls,
                         "mean_paths_in_flight": round(sst.paths_in_flight / max(sst.groups, 1))},
         "plain": {"jobs_per_s_job_at_a_time": round(plain_job, 3), "job_at_a_time_jobs": len(pj),
@@ -316,7 +320,7 @@ def context_kwargs(args, wl, trace_overlap=None):
     return dict(bvh_width=args.bvh_width, trace_block=args.trace_block, trace_chunk=args.trace_chunk,
                 leaf_max=args.leaf_max, hoist_frac=args.hoist_frac, trace_grid=args.trace_grid,
                 lds_stack=args.lds_stack, shade_waves=args.shade_waves, max_paths=args.max_paths,
-                batch_factor=args.batch_factor, trace_overlap=ov, graph=args.graph,
+                batch_factor=args.batch_factor, trace_overlap=ov, graph=max(args.graph, 0),
                 trace_xcd=args.trace_xcd or wl.get("trace_xcd", 0),
                 trace_refill=args.trace_refill or wl.get("trace_refill", 0), bvh_build=args.bvh_build)
 
@@ -555,7 +559,9 @@ def main():
     ap.add_argument("--batch-factor", type=int, default=0, help="work per batch over paths in flight (2: regeneration)")
     ap.add_argument("--trace-overlap", type=int, default=-1,
                     help="1: closest-hit and any-hit traces on two streams, 0: one stream, -1: the workload's choice")
-    ap.add_argument("--graph", type=int, default=0, help="1: bounce loop replayed from a HIP graph")
+    ap.add_argument("--graph", type=int, default=-1,
+                    help="1: bounce loop replayed from a HIP graph (-1: on for batch / serve, off for the workloads "
+                         "whose roofline rows need the kernels timed one by one)")
     ap.add_argument("--trace-xcd", type=int, default=0, help="ray-queue slices per trace launch (1: no slicing)")
     ap.add_argument("--trace-refill", type=int, default=0, help="idle lanes before a trace wave takes new rays (-1: off)")
     ap.add_argument("--leaf-max", type=int, default=0, help="triangles per BVH leaf (0: library default)")
@@ -581,7 +587,8 @@ def main():
         serve = kind == "serve"
         plain = args.plain_jobs if args.plain_jobs >= 0 else (120 if serve else 0)
         return serve_run(Y, np, serve, args.rate, args.duration, args.jobs, plain, args.frame_jobs,
-                         serve_mode=args.serve_mode, graph=args.graph, trace_overlap=max(args.trace_overlap, 0),
+                         serve_mode=args.serve_mode, graph=1 if args.graph < 0 else args.graph,
+                         trace_overlap=max(args.trace_overlap, 0),
                          tune=not args.no_tune)
 
     if args.workload in ("batch", "serve"):
