@@ -12,10 +12,12 @@
 //    shrinks and ignores t; this traversal visits children near-first and
 //    skips a box whose entry distance is beyond the best hit so far (with a
 //    1e-4 relative margin, so rounding in the box entry can never hide a
-//    triangle the reference would find).  The slab test and Moller-Trumbore
-//    use the reference's operands and order, so t, u, v and accept/reject
-//    decisions are bit-identical to the CPU restatement.  Only equal-t ties
-//    may resolve to a different triangle (test order).
+//    triangle the reference would find).  Moller-Trumbore uses the
+//    reference's operands and order, so t, u, v and the accept/reject of
+//    each triangle are bit-identical to the CPU restatement.  Boxes are
+//    tested with the reference's slab arithmetic or with a widened FMA form
+//    that enters every box the reference's arithmetic enters (yn_slab.h).
+//    Only equal-t ties may resolve to a different triangle (test order).
 //  * any hit (shadow rays, udpt.cl:307-308): true if some triangle has t in
 //    (0, tmax).
 //
