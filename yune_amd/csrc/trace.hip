@@ -170,24 +170,45 @@ __device__ __forceinline__ bool leaf_test(const float4* __restrict__ tri, uint32
     return got;
 }
 
+// The stack's two parts in explicit address spaces: through plain (generic)
+// pointers the compiler merges a pop's LDS and global branches into one
+// FLAT load, which goes through the vector-memory pipeline the node fetches
+// already saturate (and waits on both counters).
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+typedef __attribute__((address_space(1))) uint32_t gbl_u32;
+typedef __attribute__((address_space(1))) uint16_t gbl_u16;
+
 // Per-lane traversal stack: LDS for the top LST entries, global beyond.
 template <int LST, int BLOCK>
 struct Stack {
-    uint32_t* lref;         // LDS, this lane's column
-    uint16_t* lt;
-    uint32_t* gref;         // global overflow, this thread's column
-    uint16_t* gt;
+    lds_u32* lref;          // LDS, the block's arrays: entry e of column c at e * BLOCK + c
+    lds_u16* lt;
+    uint32_t col;           // this lane's column (one register for both arrays)
+    gbl_u32* gref;          // global overflow, this thread's column
+    gbl_u16* gt;
     uint32_t gstride;       // threads in the grid
     int gcap;               // overflow entries available
     uint32_t* full;         // set to 1 by a traversal that would overflow the stack (it ends there)
     int sp;
 
+    // the column, opaque to the compiler at each use: otherwise it keeps
+    // one precomputed address per array live through the whole loop, and
+    // at the kernel's register bound one of them spills
+    __device__ __forceinline__ uint32_t column() const
+    {
+        uint32_t c = col;
+        __asm__ volatile("" : "+v"(c));
+        return c;
+    }
+
     __device__ __forceinline__ bool push(uint32_t ref, float t)
     {
         uint16_t tb = (uint16_t)(__float_as_uint(t) >> 16);   // truncation: a lower bound of t >= 0
         if (sp < LST) {
-            lref[sp * BLOCK] = ref;
-            lt[sp * BLOCK] = tb;
+            const uint32_t i = sp * BLOCK + column();
+            lref[i] = ref;
+            lt[i] = tb;
         } else {
             int k = sp - LST;
             if (k >= gcap) return false;
@@ -203,8 +224,9 @@ struct Stack {
         uint32_t ref;
         uint16_t tb;
         if (sp < LST) {
-            ref = lref[sp * BLOCK];
-            tb = lt[sp * BLOCK];
+            const uint32_t i = sp * BLOCK + column();
+            ref = lref[i];
+            tb = lt[i];
         } else {
             size_t k = (size_t)(sp - LST) * gstride;
             ref = gref[k];
@@ -232,7 +254,7 @@ __device__ bool traverse(const DevScene& S, RayState& r, Stack<LST, BLOCK>& st, 
     }
     // oversized triangles hoisted out of the tree, each behind its own box
     for (int s = 0; s < S.n_hoist; ++s) {
-        float4 lo = S.hoist_box[2 * s], hi = S.hoist_box[2 * s + 1];
+        float4 lo = gload(S.hoist_box + 2 * s), hi = gload(S.hoist_box + 2 * s + 1);
         float e;
         if (!box_hit(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e) || e > r.cull)
             continue;
@@ -428,12 +450,13 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
     __shared__ uint32_t stk_all[LST * BLOCK];
     __shared__ uint16_t stkt_all[LST * BLOCK];
     Stack<LST, BLOCK> st;
-    st.lref = stk_all + threadIdx.x;
-    st.lt = stkt_all + threadIdx.x;
+    st.lref = (lds_u32*)stk_all;
+    st.lt = (lds_u16*)stkt_all;
+    st.col = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     st.gstride = gridDim.x * BLOCK;
-    st.gref = a.ovf_ref + gtid;
-    st.gt = a.ovf_t + gtid;
+    st.gref = (gbl_u32*)(a.ovf_ref + gtid);
+    st.gt = (gbl_u16*)(a.ovf_t + gtid);
     st.gcap = a.ovf_depth;
     st.full = &a.ctr->q[7][0];
     const int lane = (int)__lane_id();
@@ -556,12 +579,13 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
     __shared__ uint32_t stk_all[LST * BLOCK];
     __shared__ uint16_t stkt_all[LST * BLOCK];
     Stack<LST, BLOCK> st;
-    st.lref = stk_all + threadIdx.x;
-    st.lt = stkt_all + threadIdx.x;
+    st.lref = (lds_u32*)stk_all;
+    st.lt = (lds_u16*)stkt_all;
+    st.col = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     st.gstride = gridDim.x * BLOCK;
-    st.gref = a.ovf_ref + gtid;
-    st.gt = a.ovf_t + gtid;
+    st.gref = (gbl_u32*)(a.ovf_ref + gtid);
+    st.gt = (gbl_u16*)(a.ovf_t + gtid);
     st.gcap = a.ovf_depth;
     st.full = &a.ctr->q[7][0];
     st.sp = 0;
@@ -641,7 +665,7 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
                 }
                 if (live) {
                     for (int s = 0; s < S.n_hoist; ++s) {
-                        float4 lo = S.hoist_box[2 * s], hi = S.hoist_box[2 * s + 1];
+                        float4 lo = gload(S.hoist_box + 2 * s), hi = gload(S.hoist_box + 2 * s + 1);
                         float e;
                         if (!box_hit(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, e) ||
                             e > r.cull)
