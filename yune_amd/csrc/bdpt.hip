@@ -498,8 +498,10 @@ __global__ __launch_bounds__(256) void k_generate_bdpt(GenArgs a, DevCapture cap
         if (a.segs[mid].path_base <= p) lo = mid; else hi = mid - 1;
     }
     const GenSeg& g = a.segs[lo];
-    int q = p - g.path_base + g.first;
-    int x = g.x0 + q % g.w, y = g.y0 + q / g.w;
+    int x, y;
+    seg_pixel(p - g.path_base + g.first, g.w, g.h, x, y);
+    x += g.x0;
+    y += g.y0;
     const DevJob& J = a.jobs[g.job];
     const DevScene& S = a.scenes[J.scene];
     const yn_light_quad& L = S.light;

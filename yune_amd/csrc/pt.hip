@@ -499,8 +499,10 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a, DevCapture cap)
         if (a.segs[mid].path_base <= item) lo = mid; else hi = mid - 1;
     }
     const GenSeg& g = a.segs[lo];
-    int q = item - g.path_base + g.first;
-    int x = g.x0 + q % g.w, y = g.y0 + q / g.w;
+    int x, y;
+    seg_pixel(item - g.path_base + g.first, g.w, g.h, x, y);
+    x += g.x0;
+    y += g.y0;
     const DevJob& J = a.jobs[g.job];
     const DevScene& S = a.scenes[J.scene];
     uint32_t seed = pixel_seed(g.rand, x, y, J.W);

@@ -606,6 +606,10 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
     auto slice_lo = [&](uint32_t y) { return (uint32_t)(((uint64_t)y * n) / ns); };
     const bool all_static = (uint64_t)n <= (uint64_t)ns * ((G / ns) * wpb) * 64u;
     uint32_t y = blockIdx.x % ns, moved = 0;
+    // the current slice's bounds and wave count, recomputed only when the
+    // wave moves on (slice_lo divides 64-bit)
+    auto sreg = [](uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); };   // wave-uniform
+    uint32_t y_lo = sreg(slice_lo(y)), y_hi = sreg(slice_lo(y + 1)), y_waves = sreg(waves_of(y));
     Counts c{0, 0};
     unsigned nrays = 0;
     RayState r{};
@@ -621,18 +625,20 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
         if (more && n_idle >= refill) {
             uint32_t base = 0, lim = 0;
             for (;;) {
-                const uint32_t lo = slice_lo(y), hi = slice_lo(y + 1);
                 if (first) {
-                    base = lo + wx * 64u;            // every lane is idle on the first fetch
+                    base = y_lo + wx * 64u;          // every lane is idle on the first fetch
                     first = false;
                 } else {
                     uint32_t b = 0;
                     if (lane == 0) b = atomicAdd(a.work + y * kWorkStride, (uint32_t)n_idle);
-                    base = lo + waves_of(y) * 64u + __shfl(b, 0);
+                    base = y_lo + y_waves * 64u + __shfl(b, 0);
                 }
-                if (base < hi) { lim = hi; break; }
+                if (base < y_hi) { lim = y_hi; break; }
                 if (all_static || ++moved >= ns) { more = false; break; }
                 y = (y + 1) % ns;
+                y_lo = sreg(slice_lo(y));
+                y_hi = sreg(slice_lo(y + 1));
+                y_waves = sreg(waves_of(y));
             }
             if (all_static) more = false;            // nothing past the static rays
             const uint32_t i = base + (uint32_t)__popcll(idle & below);

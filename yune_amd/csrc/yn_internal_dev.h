@@ -33,15 +33,33 @@ struct TraceArgs {
 };
 
 // Segment of primary rays: a rectangle of one job's image at one frame.
-// Its work items are the rectangle's pixels in row order from pixel
-// `first` on (0 unless a frame's pixels are issued in several parts).
+// Its work items are the rectangle's pixels in tile order (seg_pixel) from
+// item `first` on (0 unless a frame's pixels are issued in several parts).
 struct GenSeg {
     int32_t job, slot;
     uint32_t rand;              // frame seed (kernel arg 10)
     int32_t x0, y0, w, h;
     int32_t path_base;          // first path index of the segment
-    int32_t first;              // pixel of the rectangle its first item renders
+    int32_t first;              // item of the rectangle its first path renders
 };
+
+// Item q of a w x h rectangle -> pixel (x, y) relative to its corner: bands
+// of 8 rows, each cut into tiles 8 pixels wide (the last band and the last
+// tile of a band may be smaller), row order inside a tile.  A wave's 64
+// camera rays then cover an 8 x 8 tile instead of 64 pixels of one row, and
+// fetch the same nodes more often.  A bijection on [0, w h): images do not
+// depend on it, since every sample's seed and slot follow from its pixel.
+__device__ __forceinline__ void seg_pixel(int q, int w, int h, int& x, int& y)
+{
+    const int band = q / (8 * w);
+    const int l = q - band * 8 * w;
+    const int bh = min(8, h - 8 * band);
+    const int tc = l / (8 * bh);
+    const int li = l - tc * 8 * bh;
+    const int tw = min(8, w - 8 * tc);
+    x = 8 * tc + li % tw;
+    y = 8 * band + li / tw;
+}
 
 // Threads per block of the path-tracing shade compiled for `waves` waves
 // per SIMD: two blocks fill a CU (4 SIMDs).  Its queue appends are one
