@@ -139,8 +139,8 @@ __device__ __forceinline__ void store_rec(const PathRecs& O, uint32_t ri, const 
 // camera's eye (createRay, udpt.cl:236) along the stored direction rd; a
 // bounce ray at hp + nd * EPSILON along the pending bounce direction nd
 // (udpt.cl:766, 906, 945).
-template <bool CAP>
-__device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint32_t qi, Rec& R)
+template <bool CAP, bool LM>
+__device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint32_t qi, Rec& R, lds_mat* smats)
 {
     load_rec(a.in, ext ? qi : (uint32_t)(a.in.cap - 1 - (int64_t)qi), R);
     const int4 m0 = R.m0, m1 = R.m1;
@@ -148,6 +148,8 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint
     const DevJob& J = a.jobs[jid];
     const yn_render_params& RP = J.p;
     const DevScene& S = a.scenes[(m0.y >> SCENE_SHIFT) & 0xff];     // loaded beside J, not after it
+    // LM: the launch's one scene has its materials in LDS
+    auto mat_of = [&](int id) { return LM ? load_mat(smats, id) : load_mat(S.mats, id); };
     const yn_light_quad& L = S.light;
     const bool on = RP.oren_nayar != 0;
     const bool fix = RP.fix_mis != 0;
@@ -209,7 +211,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint
             indirect = mk(0, 0, 0, 0);
             v_dirty = true;
             bounce = 0;
-            Mat m = load_mat(S.mats, vmat);
+            Mat m = mat_of(vmat);
             if (!m.spec) { st |= F_DIRECT; step = S_NEE_START; }
             else { step = S_BOUNCE_SAMPLE; }
         }
@@ -223,7 +225,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint
         } else {   // ST_BOUNCE: extension result (udpt.cl:488-529)
             float4 h = a.hit[qi];
             int htri = __float_as_int(h.y);
-            Mat m = load_mat(S.mats, vmat);
+            Mat m = mat_of(vmat);
             f4 nd = R.nd;
             float4 bnc = st4(R.bnc);
             if (htri < 0) {
@@ -234,7 +236,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint
                 f4 nhp = o + nd * h.x;
                 f4 nhn = shade_normal(S.tri_shade, htri, h.z, h.w);
                 const int nmat = tri_material(S.tri_shade, htri);
-                Mat nm = load_mat(S.mats, nmat);
+                Mat nm = mat_of(nmat);
                 indirect = indirect + tp * nm.ke;
                 if (m.spec) tp = tp * bnc.z;
                 else
@@ -269,7 +271,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint
     // makes a ray jumps past the rest.  Without a loop back-edge, values of
     // one step are not kept live through the others.
     if (step == S_NEE_START) do {   // evaluateDirectLighting (udpt.cl:535-555)
-        Mat m = load_mat(S.mats, vmat);
+        Mat m = mat_of(vmat);
         em = m.ke;
         f4 wraw;
         if (!sample_lights(L, hp, hn, seed, lpdf, wraw) || lpdf <= 0.0f) { nee_res = em; step = S_NEE_DONE; break; }
@@ -304,7 +306,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint
             code = ST_DONE;     // consumed
         }
         if (cap_on && (st & F_DIRECT) && !(st & F_CAPSH)) { cap.shadow_occ[pix] = occ; st |= F_CAPSH; }
-        Mat m = load_mat(S.mats, vmat);
+        Mat m = mat_of(vmat);
         em = m.ke;
         f4 w_o = -rd;
         bool glossy = fix ? ((st & F_NGLOSSY) != 0) : false;
@@ -336,7 +338,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint
         goto steps_done;
     } while (0);
     if (step == S_NEE_MIS) do {     // udpt.cl:599-607
-        Mat m = load_mat(S.mats, vmat);
+        Mat m = mat_of(vmat);
         em = m.ke;
         ls = R.ls;
         if (occ) { nee_res = ls + em; step = S_NEE_DONE; break; }
@@ -360,7 +362,7 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint
     } while (0);
     if (step == S_BOUNCE_SAMPLE) do {   // udpt.cl:458-487
         if (!RP.gi || bounce >= RP.max_bounces) { step = S_FINISH; break; }
-        Mat m = load_mat(S.mats, vmat);
+        Mat m = mat_of(vmat);
         float pdf = 1.0f, bprob = 0.0f, ior = 1.0f;
         bool glossy = false;
         f4 nd;
@@ -410,11 +412,18 @@ steps_done:
 // next ray goes, beside that ray.
 // CAP: the variant that records the first frame's primary hits and shadow
 // rays for the parity tests (without it, the kernel holds fewer values).
-template <int WAVES, bool CAP>
+template <int WAVES, bool CAP, bool LM>
 __global__ __launch_bounds__(shade_block_pt(WAVES), WAVES) void k_shade_pt(ShadeArgs a, DevCapture cap)
 {
     constexpr int BL = shade_block_pt(WAVES);
     __shared__ BlockReserveLds<2, BL> rsv;
+    // LM: the materials of the launch's one scene (a.n_lmats <= kLdsMats)
+    __shared__ float4 smat4[LM ? kLdsMats * 5 : 1];
+    if (LM) {
+        for (int k = (int)threadIdx.x; k < a.n_lmats * 5; k += BL) smat4[k] = a.lmats[k];
+        __syncthreads();
+    }
+    lds_mat* const smats = (lds_mat*)(void*)smat4;
     // continuous serving: ended paths per job-table entry, gathered per block
     // in a small table and added to the device counters once at the end
     // (device-scope atomics on one cache line, one per wave, serialise)
@@ -446,7 +455,7 @@ __global__ __launch_bounds__(shade_block_pt(WAVES), WAVES) void k_shade_pt(Shade
         Rec R;
         if (i < total) {
             const bool ext = i < n_ext;
-            o = advance<CAP>(a, cap, ext, ext ? i : i - n_ext, R);
+            o = advance<CAP, LM>(a, cap, ext, ext ? i : i - n_ext, R, smats);
         }
         const uint32_t n[2] = {o.kind == 1 ? 1u : 0u, o.kind == 2 ? 1u : 0u};
         uint32_t at[2];
@@ -636,12 +645,18 @@ void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t s)
 void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, int waves, hipStream_t s)
 {
     if (cap.prim_tri) {
-        hipLaunchKernelGGL((k_shade_pt<4, true>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
+        hipLaunchKernelGGL((k_shade_pt<4, true, false>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
         return;
     }
-    if (waves == 4) hipLaunchKernelGGL((k_shade_pt<4, false>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
-    else if (waves == 3) hipLaunchKernelGGL((k_shade_pt<3, false>), dim3(grid), dim3(shade_block_pt(3)), 0, s, a, cap);
-    else hipLaunchKernelGGL((k_shade_pt<2, false>), dim3(grid), dim3(shade_block_pt(2)), 0, s, a, cap);
+    const bool lm = a.lmats != nullptr && a.n_lmats <= kLdsMats;
+    if (waves == 4) {
+        if (lm) hipLaunchKernelGGL((k_shade_pt<4, false, true>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
+        else hipLaunchKernelGGL((k_shade_pt<4, false, false>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
+    } else if (waves == 3) {
+        hipLaunchKernelGGL((k_shade_pt<3, false, false>), dim3(grid), dim3(shade_block_pt(3)), 0, s, a, cap);
+    } else {
+        hipLaunchKernelGGL((k_shade_pt<2, false, false>), dim3(grid), dim3(shade_block_pt(2)), 0, s, a, cap);
+    }
 }
 void launch_resolve(const ResolveArgs& a, hipStream_t s)
 {

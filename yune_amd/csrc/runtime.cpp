@@ -478,6 +478,10 @@ void make_iter(yn_ctx* c, const LoopCfg& L, int cur, IterArgs& A)
         sa.n_zero = 2 * kMaxXcd;
         sa.ctr = c->ctr;
         sa.done_ctr = L.done_ctr;
+        if (L.single >= 0 && c->scenes[L.single].h_mats.size() <= (size_t)kLdsMats) {
+            sa.lmats = (const float4*)c->scenes[L.single].bufs[4];
+            sa.n_lmats = (int32_t)c->scenes[L.single].h_mats.size();
+        }
         sa.done_base = kMaxJobs;      // continuous serving: pieces' entries start here
     }
 }

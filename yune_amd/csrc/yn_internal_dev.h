@@ -80,7 +80,12 @@ struct ShadeArgs {
     DevCounters* ctr;
     uint32_t* done_ctr;         // continuous serving: paths ended per job-table entry from
     int32_t done_base;          //   done_base on (done_ctr[entry - done_base]), or null
+    const float4* lmats;        // the materials of the launch's one scene (5 float4 each),
+    int32_t n_lmats;            //   staged in LDS when n_lmats <= kLdsMats; or null
 };
+
+// Materials the path-tracing shade stages in LDS (80 B each).
+constexpr int kLdsMats = 64;
 
 // Shade pass of the bidirectional tracer: advances the paths of `list_in`
 // (one entry per path whose rays came back) and lists those that issued

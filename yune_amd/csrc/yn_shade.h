@@ -32,6 +32,22 @@ __device__ __forceinline__ Mat load_mat(const yn_material* __restrict__ mats, in
     return r;
 }
 
+// A scene's material table staged in LDS by the shade kernels (typed, so
+// that its reads are LDS reads, not generic loads through the vector memory
+// pipeline).
+typedef __attribute__((address_space(3))) const yn_material lds_mat;
+__device__ __forceinline__ Mat load_mat(lds_mat* mats, int id)
+{
+    lds_mat& m = mats[id];
+    Mat r;
+    r.ke = mk(m.ke.x, m.ke.y, m.ke.z, m.ke.w);
+    r.kd = mk(m.kd.x, m.kd.y, m.kd.z, m.kd.w);
+    r.ks = mk(m.ks.x, m.ks.y, m.ks.z, m.ks.w);
+    r.n = m.n; r.px = m.px; r.py = m.py; r.alpha_x = m.alpha_x;
+    r.spec = m.is_specular; r.trans = m.is_transmissive;
+    return r;
+}
+
 // reflect (udpt.cl:949-961)
 __device__ __forceinline__ f4 reflect_pt(f4 w_i, f4 normal)
 {
