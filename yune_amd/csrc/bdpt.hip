@@ -116,14 +116,16 @@ __device__ __forceinline__ void set_ray(BOut& r, f4 o, f4 d, float tmax, int id)
     r.d_p = make_float4(d.x, d.y, d.z, __int_as_float(id));
 }
 
-template <bool CAP>
-__device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int p)
+template <bool CAP, bool LM>
+__device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int p, lds_mat* smats)
 {
     const PathSoA& P = a.P;
     const BdptSoA& B = a.B;
     const DevJob& J = a.jobs[P.job[p]];
     const yn_render_params& RP = J.p;
     const DevScene& S = a.scenes[J.scene];
+    // LM: the launch's one scene has its materials in LDS
+    auto mat_of = [&](int id) { return LM ? load_mat(smats, id) : load_mat(S.mats, id); };
     const yn_light_quad& L = S.light;
     const bool fix = RP.fix_mis != 0;
     const int KB = RP.bdpt_bounces;
@@ -195,7 +197,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
                 cur.c = bc(fmaxf(0.0f, dot(pv.n, d)) / pdf);
                 cur.c = cur.c * pv.c;
             } else {
-                Mat pm = load_mat(S.mats, S.tri_mat[pv.tri]);
+                Mat pm = mat_of(S.tri_mat[pv.tri]);
                 f4 f = side == 0 ? eval_brdf(pm, -pv.d, d, pv.p, pv.n, g, prob, false, true)      // :486-490
                                  : eval_brdf(pm, d, -pv.d, pv.p, pv.n, g, prob, false, true);     // :546-550
                 cur.c = f * fmaxf(0.0f, dot(d, pv.n));
@@ -228,7 +230,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
     // subpath vertex is not kept live through them.
     if (step == T_LIGHT_NEXT || step == T_EYE_NEXT) do {   // :474-484, :532-544
         const bool light = step == T_LIGHT_NEXT;
-        Mat m = load_mat(S.mats, S.tri_mat[cur.tri]);
+        Mat m = mat_of(S.tri_mat[cur.tri]);
         float prob = 0.0f, pdf = 1.0f;
         bool g = sample_glossy_pdf(m, seed, prob, true);
         Step stop = light ? T_EYE_START : T_ROUND_START;
@@ -264,7 +266,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
         if (step == T_ROUND_START) do {        // eye vertex k: NEE start + connection rays
             if (k >= ep_len || w_eye == 0) { step = T_FINISH; break; }
             Vtx e = load_v(B, 1, k, p);
-            Mat m = load_mat(S.mats, S.tri_mat[e.tri]);
+            Mat m = mat_of(S.tri_mat[e.tri]);
             conn = 0;
             st &= ~(BF_NEE_Q | BF_NEE_DONE | BF_NGLOSSY);
             // evaluateDirectLighting start (bdpt.cl's copy of udpt.cl:535-555)
@@ -315,7 +317,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
         } while (0);
         if (step == T_NEE_SHADOW) do {         // udpt.cl:557-597 (bdpt variant of the BRDF pieces)
             Vtx e = load_v(B, 1, k, p);
-            Mat m = load_mat(S.mats, S.tri_mat[e.tri]);
+            Mat m = mat_of(S.tri_mat[e.tri]);
             if (st & BF_NEE_DONE) { nee_res = ld(*B.R.f(BF::LS, p)); step = T_CONNECT; break; }
             int occ = (st & BF_NEE_Q) ? 1 : B.occ[((size_t)p << B.shift)];
             if (cap_on && k == 0 && !(st & BF_CAPSH)) { cap.shadow_occ[pix] = occ; st |= BF_CAPSH; }
@@ -357,7 +359,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
         } while (0);
         if (step == T_NEE_MIS) do {            // udpt.cl:599-607
             Vtx e = load_v(B, 1, k, p);
-            Mat m = load_mat(S.mats, S.tri_mat[e.tri]);
+            Mat m = mat_of(S.tri_mat[e.tri]);
             f4 em = m.ke;
             f4 ls = ld(*B.R.f(BF::LS, p));
             int occ = B.occ[((size_t)p << B.shift)];
@@ -378,7 +380,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
         if (step != T_CONNECT) break;
         {            // shading :578-637 for eye vertex k
             Vtx e = load_v(B, 1, k, p);
-            Mat m = load_mat(S.mats, S.tri_mat[e.tri]);
+            Mat m = mat_of(S.tri_mat[e.tri]);
             f4 spec = m.ks, em = m.ke;
             float ks = fmaxf(fmaxf(spec.x, fmaxf(spec.y, spec.z)), 0.1f);
             f4 tp = e.c;
@@ -397,7 +399,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
                 float prob = 0.0f;
                 bool g = sample_glossy_pdf(m, seed, prob, true);
                 f4 e2l = eval_brdf(m, w_i, w_o, e.p, e.n, g, prob, false, true);
-                Mat lm = load_mat(S.mats, S.tri_mat[l.tri]);
+                Mat lm = mat_of(S.tri_mat[l.tri]);
                 g = sample_glossy_pdf(lm, seed, prob, true);
                 f4 l2e = eval_brdf(lm, -l.d, -w_i, l.p, l.n, g, prob, false, true);
                 tlp = tlp * ((e2l * gf) * l2e);
@@ -433,10 +435,17 @@ emitted:
 // waves per SIMD, 9% slower on the Cornell box.
 // CAP: the variant that records the first frame's camera hits and first
 // shadow rays for the parity tests.
-template <bool CAP>
+template <bool CAP, bool LM>
 __global__ __launch_bounds__(256, 3) void k_shade_bdpt(BdptShadeArgs a, DevCapture cap)
 {
     __shared__ BlockReserveLds<3, 256> rsv;
+    // LM: the materials of the launch's one scene (a.n_lmats <= kLdsMats)
+    __shared__ float4 smat4[LM ? kLdsMats * 5 : 1];
+    if (LM) {
+        for (int k = (int)threadIdx.x; k < a.n_lmats * 5; k += 256) smat4[k] = a.lmats[k];
+        __syncthreads();
+    }
+    lds_mat* const smats = (lds_mat*)(void*)smat4;
     const uint32_t n = *a.list_in_count;
     if (blockIdx.x == 0 && (int)threadIdx.x < a.n_zero) a.zero[threadIdx.x * kWorkStride] = 0;
     const int shift = a.B.shift;
@@ -449,7 +458,7 @@ __global__ __launch_bounds__(256, 3) void k_shade_bdpt(BdptShadeArgs a, DevCaptu
         int p = -1;
         if (i < n) {
             p = (int)a.list_in[i];
-            o = advance_bdpt<CAP>(a, cap, p);
+            o = advance_bdpt<CAP, LM>(a, cap, p, smats);
         }
         const bool more = o.ext || o.emit != 0;
         const uint32_t cnt[3] = {o.ext ? 1u : 0u, (uint32_t)__popcll(o.emit), more ? 1u : 0u};
@@ -569,8 +578,10 @@ void launch_generate_bdpt(const GenArgs& a, const DevCapture& cap, hipStream_t s
 
 void launch_shade_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int grid, hipStream_t s)
 {
-    if (cap.prim_tri) hipLaunchKernelGGL(k_shade_bdpt<true>, dim3(grid), dim3(256), 0, s, a, cap);
-    else hipLaunchKernelGGL(k_shade_bdpt<false>, dim3(grid), dim3(256), 0, s, a, cap);
+    if (cap.prim_tri) hipLaunchKernelGGL((k_shade_bdpt<true, false>), dim3(grid), dim3(256), 0, s, a, cap);
+    else if (a.lmats != nullptr && a.n_lmats <= kLdsMats)
+        hipLaunchKernelGGL((k_shade_bdpt<false, true>), dim3(grid), dim3(256), 0, s, a, cap);
+    else hipLaunchKernelGGL((k_shade_bdpt<false, false>), dim3(grid), dim3(256), 0, s, a, cap);
 }
 
 }  // namespace yune

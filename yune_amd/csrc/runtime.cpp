@@ -462,6 +462,10 @@ void make_iter(yn_ctx* c, const LoopCfg& L, int cur, IterArgs& A)
         ba.sh_out = c->ShB[nxt];
         ba.zero = &c->ctr->work[0][0][0];
         ba.n_zero = 2 * kMaxXcd;
+        if (L.single >= 0 && c->scenes[L.single].h_mats.size() <= (size_t)kLdsMats) {
+            ba.lmats = (const float4*)c->scenes[L.single].bufs[4];
+            ba.n_lmats = (int32_t)c->scenes[L.single].h_mats.size();
+        }
     } else {
         ShadeArgs& sa = A.sa;
         sa.scenes = c->d_scenes;

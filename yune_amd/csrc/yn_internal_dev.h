@@ -102,6 +102,8 @@ struct BdptShadeArgs {
     RayQueue ext_out, sh_out;
     uint32_t* zero;             // work counters to clear for the next traces
     int32_t n_zero;
+    const float4* lmats;        // the materials of the launch's one scene (5 float4 each),
+    int32_t n_lmats;            //   staged in LDS when n_lmats <= kLdsMats; or null
 };
 
 struct GenArgs {
