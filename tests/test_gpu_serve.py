@@ -93,6 +93,41 @@ def test_served_jobs_equal_jobs_alone(serve_mode, graph):
         assert c.serve_stats().rays_ext > 0
 
 
+def test_material_table_follows_scenes():
+    """Launches over several scenes read every live scene's materials from
+    one table, each scene's at its own offset; releasing a scene and making
+    another rebuilds the table and moves the offsets.  Served (several
+    scenes per launch) equals alone (one scene per launch, its own
+    materials) bit for bit after each change."""
+    import dataclasses
+    cornell, room = Y.gen_cornell(), Y.gen_icosphere_room(3, 9, 2)
+    # the Cornell box with its material list doubled: same image, more entries
+    wide = dataclasses.replace(cornell, mats=np.concatenate([cornell.mats, cornell.mats]), nodes=None)
+    p = Y.params_reference(mis=1, max_bounces=8, rr_threshold=2)
+
+    def served_equals_alone(c, pairs):
+        jobs = [c.submit(sid, s.cam, 40, 30, p, 3, 11 + k) for k, (sid, s) in enumerate(pairs)]
+        for _ in range(1000):
+            if c.serve(4) == 0:
+                break
+        for k, ((sid, s), jb) in enumerate(zip(pairs, jobs)):
+            img = jb.read()
+            jb.release()
+            alone = c.job(sid, s.cam, 40, 30, p, base_seed=11 + k)
+            alone.render(3)
+            ref = alone.read()
+            alone.release()
+            assert np.array_equal(img.view(np.int32), ref.view(np.int32)), k
+
+    with Y.Context(max_paths=1 << 15) as c:
+        a = c.scene(wide)              # slot 0: the room's offset is 2n
+        b = c.scene(room)
+        served_equals_alone(c, [(a, wide), (b, room)])
+        c.release_scene(a)
+        d = c.scene(cornell)           # slot 0 again: the room's offset is n
+        served_equals_alone(c, [(d, cornell), (b, room)])
+
+
 def test_admission_waits_for_memory():
     s = Y.gen_cornell()
     W, H = 64, 64

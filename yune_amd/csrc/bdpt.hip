@@ -116,7 +116,7 @@ __device__ __forceinline__ void set_ray(BOut& r, f4 o, f4 d, float tmax, int id)
     r.d_p = make_float4(d.x, d.y, d.z, __int_as_float(id));
 }
 
-template <bool CAP, bool LM>
+template <bool CAP, int LM>
 __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int p, lds_mat* smats)
 {
     const PathSoA& P = a.P;
@@ -124,8 +124,10 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
     const DevJob& J = a.jobs[P.job[p]];
     const yn_render_params& RP = J.p;
     const DevScene& S = a.scenes[J.scene];
-    // LM: the launch's one scene has its materials in LDS
-    auto mat_of = [&](int id) { return LM ? load_mat(smats, id) : load_mat(S.mats, id); };
+    // LM: materials in LDS, the launch's one scene's (1) or every scene's (2)
+    auto mat_of = [&](int id) {
+        return LM == 1 ? load_mat(smats, id) : LM == 2 ? load_mat(smats, S.mat_base + id) : load_mat(S.mats, id);
+    };
     const yn_light_quad& L = S.light;
     const bool fix = RP.fix_mis != 0;
     const int KB = RP.bdpt_bounces;
@@ -435,11 +437,11 @@ emitted:
 // waves per SIMD, 9% slower on the Cornell box.
 // CAP: the variant that records the first frame's camera hits and first
 // shadow rays for the parity tests.
-template <bool CAP, bool LM>
+template <bool CAP, int LM>
 __global__ __launch_bounds__(256, 3) void k_shade_bdpt(BdptShadeArgs a, DevCapture cap)
 {
     __shared__ BlockReserveLds<3, 256> rsv;
-    // LM: the materials of the launch's one scene (a.n_lmats <= kLdsMats)
+    // LM: the materials of a.lmats (a.n_lmats <= kLdsMats)
     __shared__ float4 smat4[LM ? kLdsMats * 5 : 1];
     if (LM) {
         for (int k = (int)threadIdx.x; k < a.n_lmats * 5; k += 256) smat4[k] = a.lmats[k];
@@ -578,10 +580,11 @@ void launch_generate_bdpt(const GenArgs& a, const DevCapture& cap, hipStream_t s
 
 void launch_shade_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int grid, hipStream_t s)
 {
-    if (cap.prim_tri) hipLaunchKernelGGL((k_shade_bdpt<true, false>), dim3(grid), dim3(256), 0, s, a, cap);
-    else if (a.lmats != nullptr && a.n_lmats <= kLdsMats)
-        hipLaunchKernelGGL((k_shade_bdpt<false, true>), dim3(grid), dim3(256), 0, s, a, cap);
-    else hipLaunchKernelGGL((k_shade_bdpt<false, false>), dim3(grid), dim3(256), 0, s, a, cap);
+    const int lm = a.lmats != nullptr && a.n_lmats <= kLdsMats ? a.lmode : 0;
+    if (cap.prim_tri) hipLaunchKernelGGL((k_shade_bdpt<true, 0>), dim3(grid), dim3(256), 0, s, a, cap);
+    else if (lm == 1) hipLaunchKernelGGL((k_shade_bdpt<false, 1>), dim3(grid), dim3(256), 0, s, a, cap);
+    else if (lm == 2) hipLaunchKernelGGL((k_shade_bdpt<false, 2>), dim3(grid), dim3(256), 0, s, a, cap);
+    else hipLaunchKernelGGL((k_shade_bdpt<false, 0>), dim3(grid), dim3(256), 0, s, a, cap);
 }
 
 }  // namespace yune

@@ -139,7 +139,7 @@ __device__ __forceinline__ void store_rec(const PathRecs& O, uint32_t ri, const 
 // camera's eye (createRay, udpt.cl:236) along the stored direction rd; a
 // bounce ray at hp + nd * EPSILON along the pending bounce direction nd
 // (udpt.cl:766, 906, 945).
-template <bool CAP, bool LM>
+template <bool CAP, int LM>
 __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint32_t qi, Rec& R, lds_mat* smats)
 {
     load_rec(a.in, ext ? qi : (uint32_t)(a.in.cap - 1 - (int64_t)qi), R);
@@ -148,8 +148,10 @@ __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint
     const DevJob& J = a.jobs[jid];
     const yn_render_params& RP = J.p;
     const DevScene& S = a.scenes[(m0.y >> SCENE_SHIFT) & 0xff];     // loaded beside J, not after it
-    // LM: the launch's one scene has its materials in LDS
-    auto mat_of = [&](int id) { return LM ? load_mat(smats, id) : load_mat(S.mats, id); };
+    // LM: materials in LDS, the launch's one scene's (1) or every scene's (2)
+    auto mat_of = [&](int id) {
+        return LM == 1 ? load_mat(smats, id) : LM == 2 ? load_mat(smats, S.mat_base + id) : load_mat(S.mats, id);
+    };
     const yn_light_quad& L = S.light;
     const bool on = RP.oren_nayar != 0;
     const bool fix = RP.fix_mis != 0;
@@ -412,12 +414,12 @@ steps_done:
 // next ray goes, beside that ray.
 // CAP: the variant that records the first frame's primary hits and shadow
 // rays for the parity tests (without it, the kernel holds fewer values).
-template <int WAVES, bool CAP, bool LM>
+template <int WAVES, bool CAP, int LM>
 __global__ __launch_bounds__(shade_block_pt(WAVES), WAVES) void k_shade_pt(ShadeArgs a, DevCapture cap)
 {
     constexpr int BL = shade_block_pt(WAVES);
     __shared__ BlockReserveLds<2, BL> rsv;
-    // LM: the materials of the launch's one scene (a.n_lmats <= kLdsMats)
+    // LM: the materials of a.lmats (a.n_lmats <= kLdsMats)
     __shared__ float4 smat4[LM ? kLdsMats * 5 : 1];
     if (LM) {
         for (int k = (int)threadIdx.x; k < a.n_lmats * 5; k += BL) smat4[k] = a.lmats[k];
@@ -645,17 +647,18 @@ void launch_generate(const GenArgs& a, const DevCapture& cap, hipStream_t s)
 void launch_shade_pt(const ShadeArgs& a, const DevCapture& cap, int grid, int waves, hipStream_t s)
 {
     if (cap.prim_tri) {
-        hipLaunchKernelGGL((k_shade_pt<4, true, false>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
+        hipLaunchKernelGGL((k_shade_pt<4, true, 0>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
         return;
     }
-    const bool lm = a.lmats != nullptr && a.n_lmats <= kLdsMats;
+    const int lm = a.lmats != nullptr && a.n_lmats <= kLdsMats ? a.lmode : 0;
     if (waves == 4) {
-        if (lm) hipLaunchKernelGGL((k_shade_pt<4, false, true>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
-        else hipLaunchKernelGGL((k_shade_pt<4, false, false>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
+        if (lm == 1) hipLaunchKernelGGL((k_shade_pt<4, false, 1>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
+        else if (lm == 2) hipLaunchKernelGGL((k_shade_pt<4, false, 2>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
+        else hipLaunchKernelGGL((k_shade_pt<4, false, 0>), dim3(grid), dim3(shade_block_pt(4)), 0, s, a, cap);
     } else if (waves == 3) {
-        hipLaunchKernelGGL((k_shade_pt<3, false, false>), dim3(grid), dim3(shade_block_pt(3)), 0, s, a, cap);
+        hipLaunchKernelGGL((k_shade_pt<3, false, 0>), dim3(grid), dim3(shade_block_pt(3)), 0, s, a, cap);
     } else {
-        hipLaunchKernelGGL((k_shade_pt<2, false, false>), dim3(grid), dim3(shade_block_pt(2)), 0, s, a, cap);
+        hipLaunchKernelGGL((k_shade_pt<2, false, 0>), dim3(grid), dim3(shade_block_pt(2)), 0, s, a, cap);
     }
 }
 void launch_resolve(const ResolveArgs& a, hipStream_t s)

@@ -134,6 +134,8 @@ struct yn_ctx {
     std::vector<SceneRec> scenes;
     std::vector<JobRec> jobs;
     DevScene* d_scenes = nullptr;
+    yn_material* d_mtab = nullptr;    // every live scene's materials (when at most kLdsMats)
+    int32_t n_mtab = 0;
     DevJob* d_jobs = nullptr;
     // wavefront buffers
     int64_t cap_paths = 0;
@@ -409,6 +411,24 @@ int loop_cfg(yn_ctx* c, bool bdpt, const std::vector<int>& scene_ids, bool count
     return 0;
 }
 
+// The materials the shade kernels stage in LDS: the one scene's of a
+// single-scene launch, else the table of every live scene's, when they are
+// few enough (mode 1 / 2; 0: none, materials are read from each scene's array).
+void lds_materials(const yn_ctx* c, const LoopCfg& L, const float4*& mats, int32_t& n, int32_t& mode)
+{
+    mats = nullptr;
+    n = mode = 0;
+    if (L.single >= 0 && c->scenes[L.single].h_mats.size() <= (size_t)kLdsMats) {
+        mats = (const float4*)c->scenes[L.single].bufs[4];
+        n = (int32_t)c->scenes[L.single].h_mats.size();
+        mode = 1;
+    } else if (c->d_mtab && c->n_mtab <= kLdsMats) {
+        mats = (const float4*)c->d_mtab;
+        n = c->n_mtab;
+        mode = 2;
+    }
+}
+
 void make_iter(yn_ctx* c, const LoopCfg& L, int cur, IterArgs& A)
 {
     std::memset(&A, 0, sizeof A);
@@ -462,10 +482,7 @@ void make_iter(yn_ctx* c, const LoopCfg& L, int cur, IterArgs& A)
         ba.sh_out = c->ShB[nxt];
         ba.zero = &c->ctr->work[0][0][0];
         ba.n_zero = 2 * kMaxXcd;
-        if (L.single >= 0 && c->scenes[L.single].h_mats.size() <= (size_t)kLdsMats) {
-            ba.lmats = (const float4*)c->scenes[L.single].bufs[4];
-            ba.n_lmats = (int32_t)c->scenes[L.single].h_mats.size();
-        }
+        lds_materials(c, L, ba.lmats, ba.n_lmats, ba.lmode);
     } else {
         ShadeArgs& sa = A.sa;
         sa.scenes = c->d_scenes;
@@ -482,10 +499,7 @@ void make_iter(yn_ctx* c, const LoopCfg& L, int cur, IterArgs& A)
         sa.n_zero = 2 * kMaxXcd;
         sa.ctr = c->ctr;
         sa.done_ctr = L.done_ctr;
-        if (L.single >= 0 && c->scenes[L.single].h_mats.size() <= (size_t)kLdsMats) {
-            sa.lmats = (const float4*)c->scenes[L.single].bufs[4];
-            sa.n_lmats = (int32_t)c->scenes[L.single].h_mats.size();
-        }
+        lds_materials(c, L, sa.lmats, sa.n_lmats, sa.lmode);
         sa.done_base = kMaxJobs;      // continuous serving: pieces' entries start here
     }
 }
@@ -1212,6 +1226,30 @@ static bool fma_slab_ok(const GpuLayout& L)
     return true;
 }
 
+// The table of every live scene's materials, in scene order, and each
+// scene's mat_base in it.  A shade launch over several scenes stages the
+// table in LDS when it has at most kLdsMats entries (else it is not made).
+int material_table(yn_ctx* c)
+{
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    std::vector<yn_material> all;
+    for (int i = 0; i < kMaxScenes; ++i) {
+        SceneRec& S = c->scenes[i];
+        if (!S.live) continue;
+        S.dev.mat_base = (int32_t)all.size();
+        all.insert(all.end(), S.h_mats.begin(), S.h_mats.end());
+        HIPCK(c, hipMemcpy(c->d_scenes + i, &S.dev, sizeof(DevScene), hipMemcpyHostToDevice));
+    }
+    if (c->d_mtab) hipFree(c->d_mtab);
+    c->d_mtab = nullptr;
+    c->n_mtab = (int32_t)all.size();
+    if (!all.empty() && all.size() <= (size_t)kLdsMats) {
+        HIPCK(c, hipMalloc(&c->d_mtab, all.size() * sizeof(yn_material)));
+        HIPCK(c, hipMemcpy(c->d_mtab, all.data(), all.size() * sizeof(yn_material), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
 int build_scene(yn_ctx* c, int sid, int width)
 {
     SceneRec& S = c->scenes[sid];
@@ -1410,6 +1448,7 @@ void yn_destroy(yn_ctx* c)
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     free_all(c);
     if (c->d_segs) hipFree(c->d_segs);
+    if (c->d_mtab) hipFree(c->d_mtab);
     if (c->ovf_ref) hipFree(c->ovf_ref);
     if (c->ovf_t) hipFree(c->ovf_t);
     if (c->h_ctr) hipHostFree(c->h_ctr);
@@ -1450,7 +1489,7 @@ int yn_scene_create(yn_ctx* c, const yn_triangle* tris, int32_t n_tri, const yn_
     if (int r = build_scene(c, sid, c->cfg.bvh_width)) { S = SceneRec(); return r; }
     S.live = true;
     *scene_id = sid;
-    return 0;
+    return material_table(c);
 }
 
 int yn_scene_release(yn_ctx* c, int32_t sid)
@@ -1466,7 +1505,7 @@ int yn_scene_release(yn_ctx* c, int32_t sid)
     HIPCK(c, hipStreamSynchronize(c->stream));
     for (void*& b : c->scenes[sid].bufs) { if (b) hipFree(b); b = nullptr; }
     c->scenes[sid] = SceneRec();
-    return 0;
+    return material_table(c);
 }
 
 int yn_scene_info_get(yn_ctx* c, int32_t sid, yn_scene_info* out)

@@ -31,6 +31,8 @@ struct DevScene {
     int32_t n_hoist;            // slots [0, n_hoist) are tested before traversal
     int32_t fma_slab;           // 1: every child box coordinate is finite and within
                                 //    2^60 (the traversal's FMA slab form cannot overflow)
+    int32_t mat_base;           // first entry of the scene's materials in the context's
+                                //    table of every live scene's materials
 };
 
 // Per-job constants (a job = one image being accumulated).

@@ -80,12 +80,13 @@ struct ShadeArgs {
     DevCounters* ctr;
     uint32_t* done_ctr;         // continuous serving: paths ended per job-table entry from
     int32_t done_base;          //   done_base on (done_ctr[entry - done_base]), or null
-    const float4* lmats;        // the materials of the launch's one scene (5 float4 each),
-    int32_t n_lmats;            //   staged in LDS when n_lmats <= kLdsMats; or null
+    const float4* lmats;        // materials staged in LDS (5 float4 each, n_lmats <= kLdsMats):
+    int32_t n_lmats;            //   lmode 1: the launch's one scene's, 2: every live scene's
+    int32_t lmode;              //   (a scene's at its mat_base); 0: none
 };
 
-// Materials the path-tracing shade stages in LDS (80 B each).
-constexpr int kLdsMats = 64;
+// Materials the shade kernels stage in LDS (80 B each).
+constexpr int kLdsMats = 256;
 
 // Shade pass of the bidirectional tracer: advances the paths of `list_in`
 // (one entry per path whose rays came back) and lists those that issued
@@ -102,8 +103,9 @@ struct BdptShadeArgs {
     RayQueue ext_out, sh_out;
     uint32_t* zero;             // work counters to clear for the next traces
     int32_t n_zero;
-    const float4* lmats;        // the materials of the launch's one scene (5 float4 each),
-    int32_t n_lmats;            //   staged in LDS when n_lmats <= kLdsMats; or null
+    const float4* lmats;        // materials staged in LDS (5 float4 each, n_lmats <= kLdsMats):
+    int32_t n_lmats;            //   lmode 1: the launch's one scene's, 2: every live scene's
+    int32_t lmode;              //   (a scene's at its mat_base); 0: none
 };
 
 struct GenArgs {
