@@ -117,18 +117,18 @@ __device__ __forceinline__ void set_ray(BOut& r, f4 o, f4 d, float tmax, int id)
 }
 
 template <bool CAP, int LM>
-__device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int p, lds_mat* smats)
+__device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int p, const LdsScene& lds)
 {
     const PathSoA& P = a.P;
     const BdptSoA& B = a.B;
     const DevJob& J = a.jobs[P.job[p]];
     const yn_render_params& RP = J.p;
     const DevScene& S = a.scenes[J.scene];
-    // LM: materials in LDS, the launch's one scene's (1) or every scene's (2)
+    // LM: materials and light in LDS, the launch's one scene's (1) or every scene's (2)
     auto mat_of = [&](int id) {
-        return LM == 1 ? load_mat(smats, id) : LM == 2 ? load_mat(smats, S.mat_base + id) : load_mat(S.mats, id);
+        return LM == 1 ? load_mat(lds.mats, id) : LM == 2 ? load_mat(lds.mats, S.mat_base + id) : load_mat(S.mats, id);
     };
-    const yn_light_quad& L = S.light;
+    auto& L = light_of<LM>(S.light, lds, J.scene);
     const bool fix = RP.fix_mis != 0;
     const int KB = RP.bdpt_bounces;
 
@@ -441,13 +441,16 @@ template <bool CAP, int LM>
 __global__ __launch_bounds__(256, 3) void k_shade_bdpt(BdptShadeArgs a, DevCapture cap)
 {
     __shared__ BlockReserveLds<3, 256> rsv;
-    // LM: the materials of a.lmats (a.n_lmats <= kLdsMats)
+    // LM: the materials of a.lmats (a.n_lmats <= kLdsMats) and the light quads
     __shared__ float4 smat4[LM ? kLdsMats * 5 : 1];
+    __shared__ float4 slight4[LM ? kLdsLights * 5 : 1];
     if (LM) {
         for (int k = (int)threadIdx.x; k < a.n_lmats * 5; k += 256) smat4[k] = a.lmats[k];
+        for (int k = (int)threadIdx.x; k < a.n_lights * 5; k += 256)
+            slight4[k] = reinterpret_cast<const float4*>(&a.scenes[LM == 1 ? a.lscene : k / 5].light)[k % 5];
         __syncthreads();
     }
-    lds_mat* const smats = (lds_mat*)(void*)smat4;
+    const LdsScene lds{(lds_mat*)(void*)smat4, (lds_light*)(void*)slight4};
     const uint32_t n = *a.list_in_count;
     if (blockIdx.x == 0 && (int)threadIdx.x < a.n_zero) a.zero[threadIdx.x * kWorkStride] = 0;
     const int shift = a.B.shift;
@@ -460,7 +463,7 @@ __global__ __launch_bounds__(256, 3) void k_shade_bdpt(BdptShadeArgs a, DevCaptu
         int p = -1;
         if (i < n) {
             p = (int)a.list_in[i];
-            o = advance_bdpt<CAP, LM>(a, cap, p, smats);
+            o = advance_bdpt<CAP, LM>(a, cap, p, lds);
         }
         const bool more = o.ext || o.emit != 0;
         const uint32_t cnt[3] = {o.ext ? 1u : 0u, (uint32_t)__popcll(o.emit), more ? 1u : 0u};

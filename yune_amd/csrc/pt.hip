@@ -140,19 +140,20 @@ __device__ __forceinline__ void store_rec(const PathRecs& O, uint32_t ri, const 
 // bounce ray at hp + nd * EPSILON along the pending bounce direction nd
 // (udpt.cl:766, 906, 945).
 template <bool CAP, int LM>
-__device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint32_t qi, Rec& R, lds_mat* smats)
+__device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint32_t qi, Rec& R, const LdsScene& lds)
 {
     load_rec(a.in, ext ? qi : (uint32_t)(a.in.cap - 1 - (int64_t)qi), R);
     const int4 m0 = R.m0, m1 = R.m1;
     const int jid = m1.y;
     const DevJob& J = a.jobs[jid];
     const yn_render_params& RP = J.p;
-    const DevScene& S = a.scenes[(m0.y >> SCENE_SHIFT) & 0xff];     // loaded beside J, not after it
-    // LM: materials in LDS, the launch's one scene's (1) or every scene's (2)
+    const int sid = (m0.y >> SCENE_SHIFT) & 0xff;
+    const DevScene& S = a.scenes[sid];                                 // loaded beside J, not after it
+    // LM: materials and light in LDS, the launch's one scene's (1) or every scene's (2)
     auto mat_of = [&](int id) {
-        return LM == 1 ? load_mat(smats, id) : LM == 2 ? load_mat(smats, S.mat_base + id) : load_mat(S.mats, id);
+        return LM == 1 ? load_mat(lds.mats, id) : LM == 2 ? load_mat(lds.mats, S.mat_base + id) : load_mat(S.mats, id);
     };
-    const yn_light_quad& L = S.light;
+    auto& L = light_of<LM>(S.light, lds, sid);
     const bool on = RP.oren_nayar != 0;
     const bool fix = RP.fix_mis != 0;
 
@@ -419,13 +420,16 @@ __global__ __launch_bounds__(shade_block_pt(WAVES), WAVES) void k_shade_pt(Shade
 {
     constexpr int BL = shade_block_pt(WAVES);
     __shared__ BlockReserveLds<2, BL> rsv;
-    // LM: the materials of a.lmats (a.n_lmats <= kLdsMats)
+    // LM: the materials of a.lmats (a.n_lmats <= kLdsMats) and the light quads
     __shared__ float4 smat4[LM ? kLdsMats * 5 : 1];
+    __shared__ float4 slight4[LM ? kLdsLights * 5 : 1];
     if (LM) {
         for (int k = (int)threadIdx.x; k < a.n_lmats * 5; k += BL) smat4[k] = a.lmats[k];
+        for (int k = (int)threadIdx.x; k < a.n_lights * 5; k += BL)
+            slight4[k] = reinterpret_cast<const float4*>(&a.scenes[LM == 1 ? a.lscene : k / 5].light)[k % 5];
         __syncthreads();
     }
-    lds_mat* const smats = (lds_mat*)(void*)smat4;
+    const LdsScene lds{(lds_mat*)(void*)smat4, (lds_light*)(void*)slight4};
     // continuous serving: ended paths per job-table entry, gathered per block
     // in a small table and added to the device counters once at the end
     // (device-scope atomics on one cache line, one per wave, serialise)
@@ -457,7 +461,7 @@ __global__ __launch_bounds__(shade_block_pt(WAVES), WAVES) void k_shade_pt(Shade
         Rec R;
         if (i < total) {
             const bool ext = i < n_ext;
-            o = advance<CAP, LM>(a, cap, ext, ext ? i : i - n_ext, R, smats);
+            o = advance<CAP, LM>(a, cap, ext, ext ? i : i - n_ext, R, lds);
         }
         const uint32_t n[2] = {o.kind == 1 ? 1u : 0u, o.kind == 2 ? 1u : 0u};
         uint32_t at[2];

@@ -411,21 +411,29 @@ int loop_cfg(yn_ctx* c, bool bdpt, const std::vector<int>& scene_ids, bool count
     return 0;
 }
 
-// The materials the shade kernels stage in LDS: the one scene's of a
-// single-scene launch, else the table of every live scene's, when they are
-// few enough (mode 1 / 2; 0: none, materials are read from each scene's array).
-void lds_materials(const yn_ctx* c, const LoopCfg& L, const float4*& mats, int32_t& n, int32_t& mode)
+// What the shade kernels stage in LDS: the materials and light quad of a
+// single-scene launch's scene (mode 1), else the table of every live
+// scene's materials, when it is small enough, and the light quads of the
+// scene slots up to the last live one (mode 2); 0: nothing, each path reads
+// its scene's arrays.
+template <typename A>
+void lds_materials(const yn_ctx* c, const LoopCfg& L, A& a)
 {
-    mats = nullptr;
-    n = mode = 0;
+    static_assert(kMaxScenes <= kLdsLights, "every scene slot's light quad fits the LDS table");
+    a.lmats = nullptr;
+    a.n_lmats = a.lmode = a.lscene = a.n_lights = 0;
     if (L.single >= 0 && c->scenes[L.single].h_mats.size() <= (size_t)kLdsMats) {
-        mats = (const float4*)c->scenes[L.single].bufs[4];
-        n = (int32_t)c->scenes[L.single].h_mats.size();
-        mode = 1;
+        a.lmats = (const float4*)c->scenes[L.single].bufs[4];
+        a.n_lmats = (int32_t)c->scenes[L.single].h_mats.size();
+        a.lmode = 1;
+        a.lscene = L.single;
+        a.n_lights = 1;
     } else if (c->d_mtab && c->n_mtab <= kLdsMats) {
-        mats = (const float4*)c->d_mtab;
-        n = c->n_mtab;
-        mode = 2;
+        a.lmats = (const float4*)c->d_mtab;
+        a.n_lmats = c->n_mtab;
+        a.lmode = 2;
+        for (int i = 0; i < kMaxScenes; ++i)
+            if (c->scenes[i].live) a.n_lights = i + 1;
     }
 }
 
@@ -482,7 +490,7 @@ void make_iter(yn_ctx* c, const LoopCfg& L, int cur, IterArgs& A)
         ba.sh_out = c->ShB[nxt];
         ba.zero = &c->ctr->work[0][0][0];
         ba.n_zero = 2 * kMaxXcd;
-        lds_materials(c, L, ba.lmats, ba.n_lmats, ba.lmode);
+        lds_materials(c, L, ba);
     } else {
         ShadeArgs& sa = A.sa;
         sa.scenes = c->d_scenes;
@@ -499,7 +507,7 @@ void make_iter(yn_ctx* c, const LoopCfg& L, int cur, IterArgs& A)
         sa.n_zero = 2 * kMaxXcd;
         sa.ctr = c->ctr;
         sa.done_ctr = L.done_ctr;
-        lds_materials(c, L, sa.lmats, sa.n_lmats, sa.lmode);
+        lds_materials(c, L, sa);
         sa.done_base = kMaxJobs;      // continuous serving: pieces' entries start here
     }
 }

@@ -82,11 +82,15 @@ struct ShadeArgs {
     int32_t done_base;          //   done_base on (done_ctr[entry - done_base]), or null
     const float4* lmats;        // materials staged in LDS (5 float4 each, n_lmats <= kLdsMats):
     int32_t n_lmats;            //   lmode 1: the launch's one scene's, 2: every live scene's
-    int32_t lmode;              //   (a scene's at its mat_base); 0: none
+    int32_t lmode;              //   (a scene's at its mat_base); 0: none.  Light quads likewise:
+    int32_t lscene;             //   lmode 1: scene lscene's, 2: those of scene slots [0, n_lights)
+    int32_t n_lights;
 };
 
-// Materials the shade kernels stage in LDS (80 B each).
+// Materials and light quads (scene slots) the shade kernels stage in LDS
+// (80 B each).
 constexpr int kLdsMats = 256;
+constexpr int kLdsLights = 64;
 
 // Shade pass of the bidirectional tracer: advances the paths of `list_in`
 // (one entry per path whose rays came back) and lists those that issued
@@ -105,7 +109,9 @@ struct BdptShadeArgs {
     int32_t n_zero;
     const float4* lmats;        // materials staged in LDS (5 float4 each, n_lmats <= kLdsMats):
     int32_t n_lmats;            //   lmode 1: the launch's one scene's, 2: every live scene's
-    int32_t lmode;              //   (a scene's at its mat_base); 0: none
+    int32_t lmode;              //   (a scene's at its mat_base); 0: none.  Light quads likewise:
+    int32_t lscene;             //   lmode 1: scene lscene's, 2: those of scene slots [0, n_lights)
+    int32_t n_lights;
 };
 
 struct GenArgs {

@@ -32,20 +32,38 @@ __device__ __forceinline__ Mat load_mat(const yn_material* __restrict__ mats, in
     return r;
 }
 
-// A scene's material table staged in LDS by the shade kernels (typed, so
-// that its reads are LDS reads, not generic loads through the vector memory
-// pipeline).
+// Materials and light quads staged in LDS by the shade kernels (typed, so
+// that their reads are LDS reads, not generic loads through the vector
+// memory pipeline).
 typedef __attribute__((address_space(3))) const yn_material lds_mat;
+typedef __attribute__((address_space(3))) const yn_light_quad lds_light;
+__device__ __forceinline__ f4 ld(__attribute__((address_space(3))) const yn_float4& v)
+{
+    return mk(v.x, v.y, v.z, v.w);
+}
 __device__ __forceinline__ Mat load_mat(lds_mat* mats, int id)
 {
     lds_mat& m = mats[id];
     Mat r;
-    r.ke = mk(m.ke.x, m.ke.y, m.ke.z, m.ke.w);
-    r.kd = mk(m.kd.x, m.kd.y, m.kd.z, m.kd.w);
-    r.ks = mk(m.ks.x, m.ks.y, m.ks.z, m.ks.w);
+    r.ke = ld(m.ke); r.kd = ld(m.kd); r.ks = ld(m.ks);
     r.n = m.n; r.px = m.px; r.py = m.py; r.alpha_x = m.alpha_x;
     r.spec = m.is_specular; r.trans = m.is_transmissive;
     return r;
+}
+
+// What a shade kernel staged in LDS (see lds_mode in its arguments).
+struct LdsScene {
+    lds_mat* mats;
+    lds_light* lights;
+};
+
+// The light quad of scene `sid`: from the scene table (LM 0), the one
+// staged scene's (1), or the staged table of every scene slot's (2).
+template <int LM>
+__device__ __forceinline__ auto& light_of(const yn_light_quad& scene_light, const LdsScene& ls, int sid)
+{
+    if constexpr (LM == 0) return scene_light;
+    else return ls.lights[LM == 1 ? 0 : sid];
 }
 
 // reflect (udpt.cl:949-961)
@@ -243,8 +261,8 @@ __device__ __forceinline__ float power_heuristic(float w, float p1, float p2) { 
 
 // sampleLights for LIGHT_SIZE == 1 (udpt.cl:632-667).  Returns false for
 // the j == -1 case.  wi_raw: un-normalised point-to-light vector.
-__device__ __forceinline__ bool sample_lights(const yn_light_quad& L, f4 p, f4 n, uint32_t& seed, float& light_pdf,
-                                              f4& wi_raw)
+template <typename LQ>
+__device__ __forceinline__ bool sample_lights(const LQ& L, f4 p, f4 n, uint32_t& seed, float& light_pdf, f4& wi_raw)
 {
     f4 lpos = ld(L.pos), el = ld(L.edge_l), ew = ld(L.edge_w), ln = ld(L.normal);
     float r1 = next_rand(seed);
@@ -264,7 +282,8 @@ __device__ __forceinline__ bool sample_lights(const yn_light_quad& L, f4 p, f4 n
 
 // Quad light test of traceRay (udpt.cl:244-276).  On a hit returns true and
 // sets t (the new ray length).
-__device__ __forceinline__ bool quad_hit(const yn_light_quad& L, f4 o, f4 d, float len, float& t)
+template <typename LQ>
+__device__ __forceinline__ bool quad_hit(const LQ& L, f4 o, f4 d, float len, float& t)
 {
     f4 n = ld(L.normal), pos = ld(L.pos), el = ld(L.edge_l), ew = ld(L.edge_w);
     float DdotN = dot(d, n);
