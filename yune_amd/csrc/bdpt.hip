@@ -48,7 +48,7 @@ constexpr float kPinkR = 0.988f, kPinkG = 0.0588f, kPinkB = 0.7529f;   // bdpt.c
 
 struct Vtx {
     f4 p, n, d, c;
-    int tri;
+    int mat;                // material of the vertex's triangle (-1: light vertex 0)
 };
 
 using BF = BdptRecs;
@@ -66,7 +66,7 @@ __device__ __forceinline__ Vtx load_v(const BdptSoA& B, int side, int k, int p)
     float4 q = f[0];
     Vtx v;
     v.p = mk(q.x, q.y, q.z, 1.0f);
-    v.tri = __float_as_int(q.w);
+    v.mat = __float_as_int(q.w);
     v.n = ld(f[BF::kTile]);
     v.d = ld(f[2 * BF::kTile]);
     v.c = ld(f[3 * BF::kTile]);
@@ -76,7 +76,7 @@ __device__ __forceinline__ Vtx load_v(const BdptSoA& B, int side, int k, int p)
 __device__ __forceinline__ void store_v(const BdptSoA& B, int side, int k, int p, const Vtx& v)
 {
     float4* f = vfield(B, side, k, p);
-    f[0] = make_float4(v.p.x, v.p.y, v.p.z, __int_as_float(v.tri));
+    f[0] = make_float4(v.p.x, v.p.y, v.p.z, __int_as_float(v.mat));
     f[BF::kTile] = st4(v.n);
     f[2 * BF::kTile] = st4(v.d);
     f[3 * BF::kTile] = st4(v.c);
@@ -177,7 +177,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
                 cur.n = shade_normal(S.tri_shade, htri, h.z, h.w);
                 cur.d = d;
                 cur.c = mk(1.0f, 1.0f, 1.0f, 1.0f);
-                cur.tri = htri;
+                cur.mat = S.tri_mat[htri];
                 store_v(B, 1, 0, p, cur);
                 ep_len = 1;
                 step = 1 < KB ? T_EYE_NEXT : T_ROUND_START;
@@ -194,12 +194,12 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
             cur.p = o + d * h.x;
             cur.n = shade_normal(S.tri_shade, htri, h.z, h.w);
             cur.d = d;
-            cur.tri = htri;
+            cur.mat = S.tri_mat[htri];
             if (side == 0 && k == 1) {      // :453-458
                 cur.c = bc(fmaxf(0.0f, dot(pv.n, d)) / pdf);
                 cur.c = cur.c * pv.c;
             } else {
-                Mat pm = mat_of(S.tri_mat[pv.tri]);
+                Mat pm = mat_of(pv.mat);
                 f4 f = side == 0 ? eval_brdf(pm, -pv.d, d, pv.p, pv.n, g, prob, false, true)      // :486-490
                                  : eval_brdf(pm, d, -pv.d, pv.p, pv.n, g, prob, false, true);     // :546-550
                 cur.c = f * fmaxf(0.0f, dot(d, pv.n));
@@ -232,7 +232,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
     // subpath vertex is not kept live through them.
     if (step == T_LIGHT_NEXT || step == T_EYE_NEXT) do {   // :474-484, :532-544
         const bool light = step == T_LIGHT_NEXT;
-        Mat m = mat_of(S.tri_mat[cur.tri]);
+        Mat m = mat_of(cur.mat);
         float prob = 0.0f, pdf = 1.0f;
         bool g = sample_glossy_pdf(m, seed, prob, true);
         Step stop = light ? T_EYE_START : T_ROUND_START;
@@ -268,7 +268,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
         if (step == T_ROUND_START) do {        // eye vertex k: NEE start + connection rays
             if (k >= ep_len || w_eye == 0) { step = T_FINISH; break; }
             Vtx e = load_v(B, 1, k, p);
-            Mat m = mat_of(S.tri_mat[e.tri]);
+            Mat m = mat_of(e.mat);
             conn = 0;
             st &= ~(BF_NEE_Q | BF_NEE_DONE | BF_NGLOSSY);
             // evaluateDirectLighting start (bdpt.cl's copy of udpt.cl:535-555)
@@ -319,7 +319,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
         } while (0);
         if (step == T_NEE_SHADOW) do {         // udpt.cl:557-597 (bdpt variant of the BRDF pieces)
             Vtx e = load_v(B, 1, k, p);
-            Mat m = mat_of(S.tri_mat[e.tri]);
+            Mat m = mat_of(e.mat);
             if (st & BF_NEE_DONE) { nee_res = ld(*B.R.f(BF::LS, p)); step = T_CONNECT; break; }
             int occ = (st & BF_NEE_Q) ? 1 : B.occ[((size_t)p << B.shift)];
             if (cap_on && k == 0 && !(st & BF_CAPSH)) { cap.shadow_occ[pix] = occ; st |= BF_CAPSH; }
@@ -361,7 +361,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
         } while (0);
         if (step == T_NEE_MIS) do {            // udpt.cl:599-607
             Vtx e = load_v(B, 1, k, p);
-            Mat m = mat_of(S.tri_mat[e.tri]);
+            Mat m = mat_of(e.mat);
             f4 em = m.ke;
             f4 ls = ld(*B.R.f(BF::LS, p));
             int occ = B.occ[((size_t)p << B.shift)];
@@ -382,7 +382,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
         if (step != T_CONNECT) break;
         {            // shading :578-637 for eye vertex k
             Vtx e = load_v(B, 1, k, p);
-            Mat m = mat_of(S.tri_mat[e.tri]);
+            Mat m = mat_of(e.mat);
             f4 spec = m.ks, em = m.ke;
             float ks = fmaxf(fmaxf(spec.x, fmaxf(spec.y, spec.z)), 0.1f);
             f4 tp = e.c;
@@ -401,7 +401,7 @@ __device__ BOut advance_bdpt(const BdptShadeArgs& a, const DevCapture& cap, int 
                 float prob = 0.0f;
                 bool g = sample_glossy_pdf(m, seed, prob, true);
                 f4 e2l = eval_brdf(m, w_i, w_o, e.p, e.n, g, prob, false, true);
-                Mat lm = mat_of(S.tri_mat[l.tri]);
+                Mat lm = mat_of(l.mat);
                 g = sample_glossy_pdf(lm, seed, prob, true);
                 f4 l2e = eval_brdf(lm, -l.d, -w_i, l.p, l.n, g, prob, false, true);
                 tlp = tlp * ((e2l * gf) * l2e);
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(256) void k_generate_bdpt(GenArgs a, DevCapture cap
     float area = length(ld(L.edge_l)) * length(ld(L.edge_w));
     float fwd = 1.0f / area;
     v0.c = ld(L.ke) / fwd;
-    v0.tri = -1;
+    v0.mat = -1;
     store_v(B, 0, 0, p, v0);
     float pdf = 1.0f;
     f4 d = cosine_sample(pdf, v0.p, v0.n, seed);

@@ -125,7 +125,7 @@ struct BdptRecs {
         EYE_O,                  // camera ray, kept until the eye subpath starts
         EYE_D,
         VTX                     // vertex v = side * K + k (side 0: light, 1: eye) at VTX + 4 v:
-    };                          //   position (w = triangle id bits, -1: on the light), normal,
+    };                          //   position (w = its triangle's material id bits, -1: on the light), normal,
                                 //   direction of the ray that found it, contribution
     float4* base;
     int32_t fields;             // VTX + 8 K
