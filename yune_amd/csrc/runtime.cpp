@@ -1496,8 +1496,14 @@ int yn_scene_create(yn_ctx* c, const yn_triangle* tris, int32_t n_tri, const yn_
     S.h_light = *light;
     if (int r = build_scene(c, sid, c->cfg.bvh_width)) { S = SceneRec(); return r; }
     S.live = true;
+    if (int r = material_table(c)) {
+        for (void*& bf : S.bufs) { if (bf) hipFree(bf); bf = nullptr; }
+        S = SceneRec();
+        material_table(c);
+        return r;
+    }
     *scene_id = sid;
-    return material_table(c);
+    return 0;
 }
 
 int yn_scene_release(yn_ctx* c, int32_t sid)
