@@ -35,8 +35,10 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 namespace yune {
 
@@ -176,7 +178,13 @@ void derive_tree(const yn_triangle* tris, const yn_bvh_node* in, int n_node, con
 // (the SAH estimate of node visits per ray).  Closest hits do not depend on
 // the tree (every triangle stays reachable through boxes that contain it);
 // only the order among exactly tied triangles may differ.  Deterministic:
-// one thread, fixed bin arithmetic; ~1.7 s for a million triangles.
+// one thread, fixed bin arithmetic; ~1.4 s for a million triangles, then
+// three sweeps of treelet restructuring (below: summed interior area 63.0
+// -> 58.9 root areas on that scene, 17.05 -> 16.88 BVH4 node visits per
+// ray; ~1.2 s more).
+void treelet_optimize(std::vector<yn_bvh_node>& out, int passes);
+constexpr int kTreeletPasses = 3;
+
 void sah_tree(const yn_triangle* tris, int n_tri, const std::vector<uint8_t>& hoisted, int leaf_max,
               std::vector<yn_bvh_node>& out)
 {
@@ -302,6 +310,174 @@ void sah_tree(const yn_triangle* tris, int n_tri, const std::vector<uint8_t>& ho
         todo.push_back({c + 1, mid, it.hi});
         todo.push_back({c, it.lo, mid});
     }
+    treelet_optimize(out, kTreeletPasses);
+}
+
+// Treelet restructuring of a binary tree in the reference's node format
+// (Karras and Aila, "Fast Parallel Construction of High-Quality Bounding
+// Volume Hierarchies", HPG 2013).  For each interior node, bottom up, the
+// treelet below it is grown to 7 subtrees by opening the interior subtree
+// root of largest area, and rebuilt with the topology of least summed
+// interior-box area: a dynamic program over the 127 subsets of its
+// subtrees (each split taken over the sub-masks holding the subset's
+// lowest member).  Leaves and the subtrees below the treelet are kept, so
+// the SAH cost (interior areas + the fixed leaf areas) falls with the
+// interior areas alone.  `passes` sweeps over the tree; disjoint subtrees
+// below the top 256 are swept on several threads (deterministic: a sweep
+// of a subtree touches nothing outside it), the top nodes after them.
+// The tree is re-emitted depth first, children in pairs, as sah_tree does.
+void treelet_optimize(std::vector<yn_bvh_node>& out, int passes)
+{
+    struct TN { float lo[3], hi[3]; int l, r; };
+    const int n = (int)out.size();
+    if (n < 5 || out[0].vert_len > 0 || out[0].child_idx <= 0) return;
+    std::vector<TN> T(n);
+    for (int i = 0; i < n; ++i) {
+        const yn_bvh_node& s = out[i];
+        TN& t = T[i];
+        t.lo[0] = s.aabb.p_min.x; t.lo[1] = s.aabb.p_min.y; t.lo[2] = s.aabb.p_min.z;
+        t.hi[0] = s.aabb.p_max.x; t.hi[1] = s.aabb.p_max.y; t.hi[2] = s.aabb.p_max.z;
+        const bool inner = s.vert_len <= 0 && s.child_idx > 0;
+        t.l = inner ? s.child_idx : -1;
+        t.r = inner ? s.child_idx + 1 : -1;
+    }
+    auto area = [](const float* lo, const float* hi) {
+        const double dx = (double)hi[0] - lo[0], dy = (double)hi[1] - lo[1], dz = (double)hi[2] - lo[2];
+        return dx * dy + dx * dz + dy * dz;
+    };
+    auto optimize = [&](int root) {
+        int lv[7], in[6];
+        int nl = 2, ni = 1;
+        in[0] = root;
+        lv[0] = T[root].l;
+        lv[1] = T[root].r;
+        while (nl < 7) {
+            int bk = -1;
+            double ba = -1.0;
+            for (int k = 0; k < nl; ++k)
+                if (T[lv[k]].l >= 0) {
+                    const double a = area(T[lv[k]].lo, T[lv[k]].hi);
+                    if (a > ba) { ba = a; bk = k; }
+                }
+            if (bk < 0) break;
+            const int x = lv[bk];
+            in[ni++] = x;
+            lv[bk] = T[x].l;
+            lv[nl++] = T[x].r;
+        }
+        if (nl < 3) return;
+        double cur = 0.0;
+        for (int k = 0; k < ni; ++k) cur += area(T[in[k]].lo, T[in[k]].hi);
+        const int full = (1 << nl) - 1;
+        float blo[128][3], bhi[128][3];
+        double c[128];
+        uint8_t part[128];
+        for (int m = 1; m <= full; ++m) {
+            const int low = m & -m;
+            if (m == low) {
+                const TN& t = T[lv[__builtin_ctz(m)]];
+                for (int k = 0; k < 3; ++k) { blo[m][k] = t.lo[k]; bhi[m][k] = t.hi[k]; }
+                c[m] = 0.0;
+                continue;
+            }
+            const int rest = m ^ low;
+            for (int k = 0; k < 3; ++k) {
+                blo[m][k] = std::min(blo[rest][k], blo[low][k]);
+                bhi[m][k] = std::max(bhi[rest][k], bhi[low][k]);
+            }
+            double best = INFINITY;
+            int bp = low;
+            for (int q = rest;; q = (q - 1) & rest) {
+                const int pm = q | low;
+                if (pm != m) {
+                    const double v = c[pm] + c[m ^ pm];
+                    if (v < best) { best = v; bp = pm; }
+                }
+                if (q == 0) break;
+            }
+            c[m] = area(blo[m], bhi[m]) + best;
+            part[m] = (uint8_t)bp;
+        }
+        if (!(c[full] < cur * (1.0 - 1e-9))) return;
+        int next = 1;
+        // rebuild: subset m becomes node `id` (the treelet's interior ids reused)
+        auto emit = [&](auto&& self, int m, int id) -> int {
+            if ((m & (m - 1)) == 0) return lv[__builtin_ctz(m)];
+            if (id < 0) id = in[next++];
+            const int pm = part[m];
+            const int l = self(self, pm, -1), r = self(self, m ^ pm, -1);
+            TN& t = T[id];
+            t.l = l;
+            t.r = r;
+            for (int k = 0; k < 3; ++k) { t.lo[k] = blo[m][k]; t.hi[k] = bhi[m][k]; }
+            return id;
+        };
+        emit(emit, full, root);
+    };
+    // children before parents over the subtree at `top`
+    auto sweep = [&](int top) {
+        std::vector<std::pair<int, bool>> st{{top, false}};
+        while (!st.empty()) {
+            auto [x, done] = st.back();
+            st.pop_back();
+            if (T[x].l < 0) continue;
+            if (done) { optimize(x); continue; }
+            st.push_back({x, true});
+            st.push_back({T[x].r, false});
+            st.push_back({T[x].l, false});
+        }
+    };
+    for (int pass = 0; pass < passes; ++pass) {
+        // the top nodes (breadth first) until 256 subtrees hang below them
+        std::vector<int> top{0}, front;
+        size_t h = 0;
+        std::vector<int> q{0};
+        top.clear();
+        while (h < q.size() && q.size() - h < 256) {
+            const int x = q[h++];
+            if (T[x].l < 0) { front.push_back(x); continue; }
+            top.push_back(x);
+            q.push_back(T[x].l);
+            q.push_back(T[x].r);
+        }
+        for (; h < q.size(); ++h) front.push_back(q[h]);
+        const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::atomic<int> nx{0};
+        auto work = [&]() {
+            for (int k; (k = nx.fetch_add(1)) < (int)front.size();) sweep(front[k]);
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nth; ++t) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
+        for (int k = (int)top.size() - 1; k >= 0; --k) optimize(top[k]);
+    }
+    // re-emit depth first, children in pairs
+    std::vector<yn_bvh_node> res;
+    res.reserve(n);
+    yn_bvh_node inner = out[0];
+    for (int q2 = 0; q2 < 10; ++q2) inner.vert_list[q2] = -1;
+    inner.vert_len = -1;
+    res.push_back(inner);
+    std::vector<std::pair<int, int>> st{{0, 0}};    // (tree node, output index)
+    while (!st.empty()) {
+        auto [x, o] = st.back();
+        st.pop_back();
+        const TN& t = T[x];
+        yn_bvh_node nd = t.l < 0 ? out[x] : inner;
+        nd.aabb.p_min.x = t.lo[0]; nd.aabb.p_min.y = t.lo[1]; nd.aabb.p_min.z = t.lo[2]; nd.aabb.p_min.w = 0.0f;
+        nd.aabb.p_max.x = t.hi[0]; nd.aabb.p_max.y = t.hi[1]; nd.aabb.p_max.z = t.hi[2]; nd.aabb.p_max.w = 0.0f;
+        if (t.l >= 0) {
+            const int cidx = (int)res.size();
+            nd.child_idx = cidx;
+            res.push_back(inner);
+            res.push_back(inner);
+            st.push_back({t.r, cidx + 1});
+            st.push_back({t.l, cidx});
+        }
+        res[o] = nd;
+    }
+    out.swap(res);
 }
 
 }  // namespace
