@@ -93,17 +93,19 @@ def test_served_jobs_equal_jobs_alone(serve_mode, graph):
         assert c.serve_stats().rays_ext > 0
 
 
-def test_material_table_follows_scenes():
-    """Launches over several scenes read every live scene's materials from
-    one table, each scene's at its own offset; releasing a scene and making
-    another rebuilds the table and moves the offsets.  Served (several
-    scenes per launch) equals alone (one scene per launch, its own
-    materials) bit for bit after each change."""
+@pytest.mark.parametrize("mode", ["pt", "bdpt"])
+def test_material_table_follows_scenes(mode):
+    """Launches over several scenes read every live scene's materials (and
+    light quads) from tables staged in LDS, each scene's materials at its
+    own offset; releasing a scene and making another rebuilds the table and
+    moves the offsets.  Served (several scenes per launch) equals alone (one
+    scene per launch, its own materials) bit for bit after each change."""
     import dataclasses
     cornell, room = Y.gen_cornell(), Y.gen_icosphere_room(3, 9, 2)
     # the Cornell box with its material list doubled: same image, more entries
     wide = dataclasses.replace(cornell, mats=np.concatenate([cornell.mats, cornell.mats]), nodes=None)
-    p = Y.params_reference(mis=1, max_bounces=8, rr_threshold=2)
+    p = (Y.params_reference(mis=1, max_bounces=8, rr_threshold=2) if mode == "pt" else
+         Y.params_reference(Y.YN_MODE_BDPT, bdpt_bounces=4))
 
     def served_equals_alone(c, pairs):
         jobs = [c.submit(sid, s.cam, 40, 30, p, 3, 11 + k) for k, (sid, s) in enumerate(pairs)]
