@@ -583,7 +583,12 @@ int make_layout(const yn_triangle* tris, int n_tri, int n_mat, const yn_bvh_node
             if (ar > opt.hoist_frac * root_a) big.push_back({-ar, t});
         }
         std::sort(big.begin(), big.end());
-        if ((int)big.size() > kMaxHoist) big.resize(kMaxHoist);
+        // the largest, at most kMaxHoist and an eighth of the scene: a scene
+        // made mostly of big triangles (a Cornell box) is better served by
+        // a tree over them than by testing each of them before every walk
+        // (all 36 of the Cornell box hoisted: 8% slower than 4 or none)
+        const size_t cap = std::min<size_t>(kMaxHoist, std::max<size_t>(1, (size_t)n_tri / 8));
+        if (big.size() > cap) big.resize(cap);
         std::vector<int> ids;
         for (auto& b : big) ids.push_back(b.second);
         std::sort(ids.begin(), ids.end());
