@@ -56,7 +56,9 @@ typedef struct yn_config {
     int32_t leaf_max;         /* triangles per leaf at most (1..10; 0: 1 for the
                                  library's own tree, 2 for the caller's)           */
     int32_t lds_stack;        /* traversal stack entries per lane kept in LDS:
-                                 8, 16 or 32 (0: 16); deeper entries spill to HBM  */
+                                 8, 12, 16 or 32 (0: per launch, 12 when the largest
+                                 scene's nodes exceed 4 MB, else 16; the
+                                 no-allocation trace: 16); deeper entries spill to HBM  */
     int32_t shade_waves;      /* occupancy the shade kernel is compiled for: 2, 3, 4
                                  (0: 4)                                             */
     int32_t batch_factor;     /* path-tracing work per batch over paths in flight:

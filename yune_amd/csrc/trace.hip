@@ -437,12 +437,12 @@ pop:
 }  // namespace
 
 // Occupancy the kernel is compiled for, per SIMD: the LDS stack (6 B per
-// entry and lane, 160 KB per CU) allows 13 / 6 / 3 waves per SIMD for 8 /
-// 16 / 32 entries, and registers are held to what 7 / 6 / 3 waves leave
-// (72 / 80 / 168).  More waves in flight hide more of the node-fetch
-// latency; past 7 the register spills cost more than they hide.
+// entry and lane, 160 KB per CU) allows 13 / 8 / 6 / 3 waves per SIMD for
+// 8 / 12 / 16 / 32 entries, and registers are held to what 7 / 7 / 6 / 3
+// waves leave (72 / 72 / 80 / 168).  More waves in flight hide more of the
+// node-fetch latency; past 7 the register spills cost more than they hide.
 template <int LST>
-constexpr int trace_waves() { return LST == 8 ? 7 : (LST == 16 ? 6 : 3); }
+constexpr int trace_waves() { return LST <= 12 ? 7 : (LST == 16 ? 6 : 3); }
 
 template <int W, bool ANY, int LST, int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a)
@@ -747,19 +747,20 @@ template <bool ANY>
 void launch_l(const TraceArgs& a, int width, int block, int grid, bool count, int lds, hipStream_t s)
 {
     if (lds == 8) launch_w<ANY, 8>(a, width, block, grid, count, s);
+    else if (lds == 12) launch_w<ANY, 12>(a, width, block, grid, count, s);
     else if (lds == 32) launch_w<ANY, 32>(a, width, block, grid, count, s);
     else launch_w<ANY, 16>(a, width, block, grid, count, s);
 }
 }  // namespace
 
-// LDS part of the traversal stack: 8, 16 or 32 entries per lane.
-int trace_lds_stack(int lds) { return lds == 8 || lds == 32 ? lds : 16; }
+// LDS part of the traversal stack: 8, 12, 16 or 32 entries per lane.
+int trace_lds_stack(int lds) { return lds == 8 || lds == 12 || lds == 32 ? lds : 16; }
 
 // Waves per SIMD a trace launch runs at for a configured LDS stack.
 int trace_resident_waves(int lds)
 {
     lds = trace_lds_stack(lds);
-    return lds == 8 ? trace_waves<8>() : (lds == 32 ? trace_waves<32>() : trace_waves<16>());
+    return lds == 8 ? trace_waves<8>() : lds == 12 ? trace_waves<12>() : (lds == 32 ? trace_waves<32>() : trace_waves<16>());
 }
 
 // `stack`: traversal stack entries the scene can need; the launch is refused
