@@ -413,6 +413,29 @@ def test_regeneration_with_work_beyond_capacity(graph):
 
 
 @pytest.mark.parametrize("mode", [Y.YN_MODE_PT, Y.YN_MODE_BDPT])
+def test_lds_stack_sizes_keep_images(mode):
+    """The traversal stack's LDS part (8, 12, 16 or 32 entries, or chosen
+    per launch) only moves entries between LDS and the overflow area in
+    memory: images and node counts are the same bit for bit."""
+    s = Y.gen_icosphere_room(3, 9, 2)
+    W, H, F = 64, 48, 2
+    kw = dict(bdpt_bounces=4) if mode else dict(mis=1, max_bounces=8, rr_threshold=2)
+    p = Y.params_reference(mode, **kw)
+    out = []
+    for lds in (0, 8, 12, 16, 32):
+        with Y.Context(max_paths=1 << 14, lds_stack=lds) as c:
+            assert c.config()["lds_stack"] == lds
+            job = c.job(c.scene(s), s.cam, W, H, p, base_seed=29)
+            job.set_counting(True)
+            job.render(F)
+            st = job.stats()
+            out.append((job.read(), st.nodes_ext, st.nodes_shadow))
+    for img, ne, ns in out[1:]:
+        assert np.array_equal(img.view(np.int32), out[0][0].view(np.int32))
+        assert (ne, ns) == out[0][1:]
+
+
+@pytest.mark.parametrize("mode", [Y.YN_MODE_PT, Y.YN_MODE_BDPT])
 def test_trace_overlap_keeps_images(mode):
     """Closest-hit and any-hit traces on two streams (each with its own half
     of the stack-overflow area, joined before the shade) give the same image
