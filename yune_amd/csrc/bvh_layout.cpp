@@ -38,6 +38,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <system_error>
 #include <thread>
 
 namespace yune {
@@ -447,7 +448,11 @@ void treelet_optimize(std::vector<yn_bvh_node>& out, int passes)
             for (int k; (k = nx.fetch_add(1)) < (int)front.size();) sweep(front[k]);
         };
         std::vector<std::thread> th;
-        for (int t = 1; t < nth; ++t) th.emplace_back(work);
+        try {
+            for (int t = 1; t < nth; ++t) th.emplace_back(work);
+        } catch (const std::system_error&) {
+            // no more threads: the ones started and this one share the rest
+        }
         work();
         for (auto& t : th) t.join();
         for (int k = (int)top.size() - 1; k >= 0; --k) optimize(top[k]);
