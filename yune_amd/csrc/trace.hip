@@ -573,7 +573,10 @@ __device__ __forceinline__ void store_hit(const TraceArgs& a, int pidx, const fl
 // all of them.  Each ray's own steps, tests and results are those of
 // k_trace.  The first fetch of every wave is static (rays wave*64 ..),
 // later ones come after the static range from the work counter.
-template <int W, bool ANY, int LST, int BLOCK, bool COUNT>
+// SS: every ray renders the one scene a.single_scene, whose node and
+// triangle arrays are then wave-uniform (scalar registers; per-lane
+// pointers would take four vector registers from a kernel held to 72).
+template <int W, bool ANY, int LST, int BLOCK, bool COUNT, bool SS>
 __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArgs a)
 {
     __shared__ uint32_t stk_all[LST * BLOCK];
@@ -616,8 +619,9 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
     uint32_t node = 0;
     int pidx = 0;
     bool live = false, got = false, exact = false;
-    const float4* nodes = nullptr;
-    const float4* tri = nullptr;
+    const int ss = SS ? __builtin_amdgcn_readfirstlane(a.single_scene) : 0;
+    const float4* nodes = SS ? a.scenes[ss].nodes : nullptr;
+    const float4* tri = SS ? a.scenes[ss].tri_isect : nullptr;
     bool more = true, first = true;
     for (;;) {
         const unsigned long long idle = __ballot(!live);
@@ -646,8 +650,9 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
                 float4 ot = a.q.o_t[i];
                 float4 dp = a.q.d_p[i];
                 pidx = __float_as_int(dp.w);
-                int sc = a.single_scene >= 0 ? a.single_scene
-                                             : a.jobs[a.by_queue ? pidx : a.path_job[pidx >> a.id_shift]].scene;
+                int sc = SS ? ss
+                            : a.single_scene >= 0 ? a.single_scene
+                                                  : a.jobs[a.by_queue ? pidx : a.path_job[pidx >> a.id_shift]].scene;
                 if (a.by_queue) pidx = (int)i;      // the result goes beside the ray
                 const DevScene& S = a.scenes[sc];
                 exact = !start_ray(r, ot, dp, S);
@@ -657,8 +662,10 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
                 // a ray with a NaN component hits nothing (see k_trace)
                 live = ot.x == ot.x && ot.y == ot.y && ot.z == ot.z && ot.w == ot.w && dp.x == dp.x &&
                        dp.y == dp.y && dp.z == dp.z;
-                nodes = S.nodes;
-                tri = S.tri_isect;
+                if (!SS) {
+                    nodes = S.nodes;
+                    tri = S.tri_isect;
+                }
                 // the tests ahead of the tree, as in traverse(): every slot
                 // without a BVH; the hoisted slots; a root that is one leaf
                 if (live && S.brute) {
@@ -721,8 +728,18 @@ template <int W, bool ANY, int LST, int BL>
 void launch_b(const TraceArgs& a, int grid, bool count, hipStream_t s)
 {
     if (a.refill > 0) {
-        if (count) hipLaunchKernelGGL((k_trace_rf<W, ANY, LST, BL, true>), dim3(grid), dim3(BL), 0, s, a);
-        else hipLaunchKernelGGL((k_trace_rf<W, ANY, LST, BL, false>), dim3(grid), dim3(BL), 0, s, a);
+        bool done = false;
+        if constexpr (W == 4) {
+            if (a.single_scene >= 0) {
+                if (count) hipLaunchKernelGGL((k_trace_rf<W, ANY, LST, BL, true, true>), dim3(grid), dim3(BL), 0, s, a);
+                else hipLaunchKernelGGL((k_trace_rf<W, ANY, LST, BL, false, true>), dim3(grid), dim3(BL), 0, s, a);
+                done = true;
+            }
+        }
+        if (!done) {
+            if (count) hipLaunchKernelGGL((k_trace_rf<W, ANY, LST, BL, true, false>), dim3(grid), dim3(BL), 0, s, a);
+            else hipLaunchKernelGGL((k_trace_rf<W, ANY, LST, BL, false, false>), dim3(grid), dim3(BL), 0, s, a);
+        }
     } else {
         if (count) hipLaunchKernelGGL((k_trace<W, ANY, LST, BL, true>), dim3(grid), dim3(BL), 0, s, a);
         else hipLaunchKernelGGL((k_trace<W, ANY, LST, BL, false>), dim3(grid), dim3(BL), 0, s, a);
