@@ -185,8 +185,9 @@ struct Stack {
     lds_u32* lref;          // LDS, the block's arrays: entry e of column c at e * BLOCK + c
     lds_u16* lt;
     uint32_t col;           // this lane's column (one register for both arrays)
-    gbl_u32* gref;          // global overflow, this thread's column
-    gbl_u16* gt;
+    gbl_u32* gref;          // global overflow area: entry k of grid thread g at
+    gbl_u16* gt;            //   k * gstride + g, g = gbase + column()
+    uint32_t gbase;         // the block's first grid thread
     uint32_t gstride;       // threads in the grid
     int gcap;               // overflow entries available
     uint32_t* full;         // set to 1 by a traversal that would overflow the stack (it ends there)
@@ -212,8 +213,9 @@ struct Stack {
         } else {
             int k = sp - LST;
             if (k >= gcap) return false;
-            gref[(size_t)k * gstride] = ref;
-            gt[(size_t)k * gstride] = tb;
+            const size_t i = (size_t)k * gstride + gbase + column();
+            gref[i] = ref;
+            gt[i] = tb;
         }
         ++sp;
         return true;
@@ -228,9 +230,9 @@ struct Stack {
             ref = lref[i];
             tb = lt[i];
         } else {
-            size_t k = (size_t)(sp - LST) * gstride;
-            ref = gref[k];
-            tb = gt[k];
+            const size_t i = (size_t)(sp - LST) * gstride + gbase + column();
+            ref = gref[i];
+            tb = gt[i];
         }
         t = __uint_as_float((uint32_t)tb << 16);
         return ref;
@@ -453,10 +455,10 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace(TraceArgs a
     st.lref = (lds_u32*)stk_all;
     st.lt = (lds_u16*)stkt_all;
     st.col = threadIdx.x;
-    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     st.gstride = gridDim.x * BLOCK;
-    st.gref = (gbl_u32*)(a.ovf_ref + gtid);
-    st.gt = (gbl_u16*)(a.ovf_t + gtid);
+    st.gref = (gbl_u32*)a.ovf_ref;
+    st.gt = (gbl_u16*)a.ovf_t;
+    st.gbase = blockIdx.x * BLOCK;
     st.gcap = a.ovf_depth;
     st.full = &a.ctr->q[7][0];
     const int lane = (int)__lane_id();
@@ -585,10 +587,10 @@ __global__ __launch_bounds__(BLOCK, trace_waves<LST>()) void k_trace_rf(TraceArg
     st.lref = (lds_u32*)stk_all;
     st.lt = (lds_u16*)stkt_all;
     st.col = threadIdx.x;
-    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     st.gstride = gridDim.x * BLOCK;
-    st.gref = (gbl_u32*)(a.ovf_ref + gtid);
-    st.gt = (gbl_u16*)(a.ovf_t + gtid);
+    st.gref = (gbl_u32*)a.ovf_ref;
+    st.gt = (gbl_u16*)a.ovf_t;
+    st.gbase = blockIdx.x * BLOCK;
     st.gcap = a.ovf_depth;
     st.full = &a.ctr->q[7][0];
     st.sp = 0;
