@@ -86,8 +86,12 @@ __device__ __forceinline__ Out make_ray(int kind, f4 o, f4 d, float tmax, int jo
 // waiting for the state (a new path's record holds only m0, m1 and the
 // camera ray's direction; the rest is not used then); only the MIS ray's
 // fields, used by one state, wait for it.  The shade pass is bound by the
-// latency of such chains of loads.
-__device__ __forceinline__ void load_rec(const PathRecs& I, uint32_t ri, Rec& R)
+// latency of such chains of loads.  Which queue the result came from is
+// known before the state: a bounce result (extension queue) reads the
+// pending bounce and never the NEE light direction, a shadow result the
+// light direction and never the pending bounce (each is rewritten before
+// the step stores it), so each loads only its own.
+__device__ __forceinline__ void load_rec(const PathRecs& I, uint32_t ri, Rec& R, bool ext)
 {
     using F = PathRecs;
     R.m0 = *reinterpret_cast<const int4*>(I.f(F::M0, ri));
@@ -98,9 +102,13 @@ __device__ __forceinline__ void load_rec(const PathRecs& I, uint32_t ri, Rec& R)
     R.tp = ld(*I.f(F::TP, ri));
     R.direct = ld(*I.f(F::DIRECT, ri));
     R.indirect = ld(*I.f(F::INDIRECT, ri));
-    R.nd = ld(*I.f(F::ND, ri));
-    R.bnc = ld(*I.f(F::BNC, ri));
-    R.wi = ld(*I.f(F::WI, ri));
+    R.nd = R.bnc = R.wi = mk(0, 0, 0, 0);
+    if (ext) {
+        R.nd = ld(*I.f(F::ND, ri));
+        R.bnc = ld(*I.f(F::BNC, ri));
+    } else {
+        R.wi = ld(*I.f(F::WI, ri));
+    }
     if ((R.m0.y & ST_MASK) == ST_NEE_MIS) {
         R.ls = ld(*I.f(F::LS, ri));
         R.mis = ld(*I.f(F::MIS, ri));
@@ -142,7 +150,7 @@ __device__ __forceinline__ void store_rec(const PathRecs& O, uint32_t ri, const 
 template <bool CAP, int LM>
 __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint32_t qi, Rec& R, const LdsScene& lds)
 {
-    load_rec(a.in, ext ? qi : (uint32_t)(a.in.cap - 1 - (int64_t)qi), R);
+    load_rec(a.in, ext ? qi : (uint32_t)(a.in.cap - 1 - (int64_t)qi), R, ext);
     const int4 m0 = R.m0, m1 = R.m1;
     const int jid = m1.y;
     const DevJob& J = a.jobs[jid];
