@@ -110,3 +110,18 @@ def test_lower_sah_cost_than_the_reference_tree():
     s = Y.gen_icosphere_room(2, 8, 3)
     ours, ref = interior_area(Y.build_gpu_bvh(s.tris, 10)), interior_area(s.bvh())
     assert ours < 0.8 * ref, (ours, ref)
+
+
+def test_treelet_restructuring_keeps_lowering_the_area():
+    """The binned SAH build is followed by treelet restructuring, which only
+    ever replaces a treelet by one of smaller summed interior area: on a
+    room of icospheres the tree's interior area is 10.39 root areas (10.73
+    for the binned build alone), and building is deterministic."""
+    s = Y.gen_icosphere_room(5, 4, 2)
+    nodes = Y.build_gpu_bvh(s.tris, 1)
+    root = nodes[0]
+    d = (root["p_max"][:3] - root["p_min"][:3]).astype(np.float64)
+    ratio = interior_area(nodes) / float(d[0] * d[1] + d[0] * d[2] + d[1] * d[2])
+    assert ratio < 10.5, ratio
+    again = Y.build_gpu_bvh(s.tris, 1)
+    assert np.array_equal(nodes.view(np.uint8), again.view(np.uint8))
