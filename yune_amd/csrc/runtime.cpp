@@ -384,11 +384,12 @@ struct IterArgs { TraceArgs te, ts; ShadeArgs sa; BdptShadeArgs ba; };
 
 // LDS traversal-stack entries per lane of a trace launch whose largest
 // scene has `node_bytes` of nodes: the configured count, else 12 (7 waves
-// per SIMD, a few registers spilled) for trees larger than an XCD's 4 MB
-// L2, whose node fetches wait on the MALL or HBM and want more waves to
-// hide them, and 16 (6 waves, no spills) for trees that stay in cache.
-// (1M-triangle room, 60 MB of nodes: 12 entries 2.5% faster than 16;
-// Cornell box: 16 entries 2-5% faster on the traces than 12.)
+// per SIMD, registers held to 72) for trees larger than an XCD's 4 MB L2,
+// whose node fetches wait on the MALL or HBM and want more waves to hide
+// them, and 16 (6 waves, 80 registers) for trees that stay in cache.
+// (1M-triangle room, 60 MB of nodes: 12 entries 2.5% faster than 16, and
+// launches over several such scenes 1.3% faster; the Cornell box and its
+// bidirectional workload: within 1% either way.)
 int lds_for(const yn_ctx* c, size_t node_bytes)
 {
     if (c->cfg.lds_stack > 0) return c->cfg.lds_stack;
