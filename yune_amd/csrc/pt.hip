@@ -91,12 +91,16 @@ __device__ __forceinline__ Out make_ray(int kind, f4 o, f4 d, float tmax, int jo
 // pending bounce and never the NEE light direction, a shadow result the
 // light direction and never the pending bounce (each is rewritten before
 // the step stores it), so each loads only its own.
-__device__ __forceinline__ void load_rec(const PathRecs& I, uint32_t ri, Rec& R, bool ext)
+__device__ __forceinline__ void load_rec(const PathRecs& I, uint32_t ri, Rec& R, bool ext, bool primary)
 {
     using F = PathRecs;
     R.m0 = *reinterpret_cast<const int4*>(I.f(F::M0, ri));
     R.m1 = *reinterpret_cast<const int4*>(I.f(F::M1, ri));
     R.rd = ld(*I.f(F::RD, ri));
+    if (primary) {          // a camera ray's record: the rest is set by its step
+        R.hp = R.hn = R.tp = R.direct = R.indirect = R.nd = R.bnc = R.wi = mk(0, 0, 0, 0);
+        return;
+    }
     R.hp = ld(*I.f(F::HP, ri));
     R.hn = ld(*I.f(F::HN, ri));
     R.tp = ld(*I.f(F::TP, ri));
@@ -150,7 +154,7 @@ __device__ __forceinline__ void store_rec(const PathRecs& O, uint32_t ri, const 
 template <bool CAP, int LM>
 __device__ Out advance(const ShadeArgs& a, const DevCapture& cap, bool ext, uint32_t qi, Rec& R, const LdsScene& lds)
 {
-    load_rec(a.in, ext ? qi : (uint32_t)(a.in.cap - 1 - (int64_t)qi), R, ext);
+    load_rec(a.in, ext ? qi : (uint32_t)(a.in.cap - 1 - (int64_t)qi), R, ext, ext && a.ext_primary);
     const int4 m0 = R.m0, m1 = R.m1;
     const int jid = m1.y;
     const DevJob& J = a.jobs[jid];

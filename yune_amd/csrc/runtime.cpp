@@ -677,6 +677,14 @@ int run_batch(yn_ctx* c, const std::vector<Region>& regs, JobRec& stat_job, cons
         } else {
             for (int g = 0; g < group; ++g, ++it) {
                 bool timed = (int)spans.size() + 3 < kEvPool / 2;
+                if (it == 0 && !bdpt) {
+                    // the first iteration shades camera rays only: their
+                    // records hold m0, m1 and the direction
+                    IterArgs first = iters[0];
+                    first.sa.ext_primary = 1;
+                    if (int r = enqueue_iter(c, L, first, cap, timed ? &spans : nullptr)) return r;
+                    continue;
+                }
                 if (int r = enqueue_iter(c, L, iters[it & 1], cap, timed ? &spans : nullptr)) return r;
             }
         }

@@ -85,6 +85,8 @@ struct ShadeArgs {
     int32_t lmode;              //   (a scene's at its mat_base); 0: none.  Light quads likewise:
     int32_t lscene;             //   lmode 1: scene lscene's, 2: those of scene slots [0, n_lights)
     int32_t n_lights;
+    int32_t ext_primary;        // 1: every extension-queue entry is a camera ray (a batch's
+                                //    first iteration), whose record holds only m0, m1, rd
 };
 
 // Materials and light quads (scene slots) the shade kernels stage in LDS
